@@ -1,0 +1,245 @@
+/*
+ * phc.h -- C ABI of the MI355X path-tracking library (libphc_hip.so).
+ *
+ * The library tracks the paths of polyhomocont's total-degree gamma-trick
+ * homotopy on the GPU: for every start point it runs the predictor-corrector
+ * tracker from t = 1 to the start of the end game and then the Cauchy end
+ * game, and returns one endpoint record per path. Everything before the
+ * tracking (random constants, start points) and after it (classification,
+ * refinement, deduplication) stays on the host, in the Fortran layer that
+ * keeps the reference API `solve(sys, res, status, options)`.
+ *
+ * Conventions
+ *  - Complex numbers are two consecutive doubles (re, im), i.e. the layout
+ *    of Fortran complex(c_double_complex) and C99 double _Complex.
+ *  - All indices are 0-based. Arrays are caller-owned; the library copies
+ *    what it needs into its own device memory during the call.
+ *  - Return values: PHC_OK (0) or one of the PHC_ERR_* codes. Positive codes
+ *    reuse the reference status codes (src/config.f90:17-23); negative codes
+ *    are device/runtime errors. The library never falls back to a CPU path:
+ *    without a usable GPU every tracking entry point returns PHC_ERR_DEVICE.
+ *  - Every entry point is reentrant; state lives in the explicit context.
+ *
+ * Reference interfaces replaced (file:line in EVADEVacStab/polyhomocont):
+ *  - phc_track_paths        <- the per-path loop of `solve`
+ *                              (src/solve.f90:190-192: start_solution +
+ *                              endpoint) and `endpoint` itself
+ *                              (src/solve.f90:217-252) with everything it
+ *                              calls: track_path/track_segment/predict_rk4/
+ *                              tangent/newton_correct (src/tracker.f90:179-360),
+ *                              cauchy_endgame/backward_error/better/loops
+ *                              (src/endgame.f90:124-378), tdh_evaluate
+ *                              (src/homotopy.f90:188-231), eval_terms via
+ *                              sparse_evaluate_homogeneous
+ *                              (src/sparse.f90:209-280), lu_factor/lu_solve/
+ *                              norm_inf (src/linalg.f90:26-114,176-189).
+ *  - phc_track_batch        <- the same per-path loop for many independent
+ *                              `solve` calls at once (no reference
+ *                              counterpart: the reference is serial).
+ *  - phc_server_*           <- no reference counterpart: a continuous-
+ *                              batching request loop around the same kernels.
+ */
+#ifndef PHC_H
+#define PHC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+/* Library return codes; 1..3 are the reference codes (src/config.f90:17-23). */
+#define PHC_OK 0
+#define PHC_ERR_INVALID_INPUT 1
+#define PHC_ERR_SINGULAR_MATRIX 2
+#define PHC_ERR_NUMERICAL 3
+#define PHC_ERR_DEVICE (-1)      /* no usable GPU, or a HIP call failed      */
+#define PHC_ERR_CAPACITY (-2)    /* request exceeds a configured limit       */
+#define PHC_ERR_PENDING (-3)     /* server ticket not finished yet           */
+
+/* Per-path status (src/tracker.f90:45-51, src/solve.f90:53). */
+#define PHC_PATH_SUCCESS 0
+#define PHC_PATH_FAILED_MIN_STEP 1
+#define PHC_PATH_FAILED_MAX_STEPS 2
+#define PHC_PATH_ENDGAME_FAILED 3
+
+/* End game status (src/endgame.f90:64-71). */
+#define PHC_ENDGAME_CONVERGED 0
+#define PHC_ENDGAME_NOT_CONVERGED 1
+#define PHC_ENDGAME_FAILED 2
+
+/* Largest number of homotopy unknowns N = n + 1 the kernels support. */
+#define PHC_MAX_UNKNOWNS 16
+
+/* ---- options ----------------------------------------------------------- */
+/* Field-for-field mirror of the reference option types; the defaults are the
+ * reference defaults (phc_default_options). Fields are ordered doubles first
+ * so that the C layout and the Fortran bind(C) layout carry no padding. */
+
+/* tracker_options (src/tracker.f90:56-77) */
+typedef struct phc_tracker_options {
+  double step_init;      /* 0.05   */
+  double step_max;       /* 0.1    */
+  double step_min;       /* 1e-14  */
+  double corrector_tol;  /* 1e-10  */
+  double contraction;    /* 0.5    */
+  int32_t corrector_maxit; /* 3    */
+  int32_t grow_after;    /* 5      */
+  int32_t max_steps;     /* 20000  */
+  int32_t reserved;      /* 0      */
+} phc_tracker_options;
+
+/* endgame_options (src/endgame.f90:73-101) */
+typedef struct phc_endgame_options {
+  double t_start;           /* 0.1   */
+  double radius_factor;     /* 0.25  */
+  double radius_min;        /* 1e-13 */
+  double loop_radius;       /* 1e-5  */
+  double winding_agreement; /* 0.2   */
+  double tol;               /* 1e-11 */
+  double residual_tol;      /* 1e-8  */
+  double closure_factor;    /* 0.1   */
+  int32_t nsamples;         /* 8     */
+  int32_t max_winding;      /* 16    */
+} phc_endgame_options;
+
+/* solve_options (src/solve.f90:57-84) */
+typedef struct phc_solve_options {
+  phc_tracker_options tracker;
+  phc_endgame_options endgame;
+  double infinity_tol;       /* 1e-8  */
+  double singular_cond;      /* 1e10  */
+  double real_tol;           /* 1e-8  */
+  double dedup_tol;          /* 1e-8  */
+  double singular_dedup_tol; /* 1e-6  */
+  int32_t use_endgame;       /* 1 (logical .true.) */
+  int32_t seed;              /* 1     */
+  int32_t refine_maxit;      /* 5     */
+  int32_t reserved;          /* 0     */
+} phc_solve_options;
+
+/* Fills *opts with the reference defaults. */
+void phc_default_options(phc_solve_options *opts);
+
+/* ---- inputs ------------------------------------------------------------ */
+/* A sparse_system in homogenized form: the private components of the
+ * reference type (src/sparse.f90:31-48) after add_equation has dropped zero
+ * terms and appended the w exponent d_i - |a| (src/sparse.f90:119-138). */
+typedef struct phc_system {
+  int32_t n;              /* variables = equations; N = n + 1 unknowns   */
+  int32_t nterms;         /* total number of terms                       */
+  int32_t maxexp;         /* largest entry of hexps                      */
+  int32_t reserved;
+  const int32_t *first;   /* [n + 1] term offsets: equation i owns terms
+                             first[i] .. first[i+1]-1 (0-based)          */
+  const int32_t *deg;     /* [n] total degrees d_i >= 1                  */
+  const int32_t *hexps;   /* [nterms][N] homogenized exponent vectors, w
+                             last (Fortran hexps(N, nterms) column-major) */
+  const double *coefs;    /* [nterms][2] complex coefficients            */
+} phc_system;
+
+/* Random constants of total_degree_homotopy (src/homotopy.f90:78-100),
+ * drawn on the host (src/homotopy.f90:135-146). */
+typedef struct phc_homotopy {
+  const double *gamma;    /* [2]                                         */
+  const double *b;        /* [n][2] start system constants, |b_i| = 1    */
+  const double *patch;    /* [N][2] affine patch a, unit 2-norm          */
+} phc_homotopy;
+
+/* ---- outputs ----------------------------------------------------------- */
+/* One record per path: what `endpoint` leaves in path_result
+ * (src/solve.f90:217-252) plus the endpoint z and the end game's own
+ * residual. Struct of arrays, each of length npaths; pointers marked
+ * optional may be NULL. */
+typedef struct phc_endpoints {
+  int32_t *status;      /* PHC_PATH_*                                       */
+  double *t;            /* real t reached (0 after a successful end game)   */
+  double *z;            /* [npaths][N][2]: end game estimate on success,
+                           else the last point tracked                      */
+  int32_t *winding;     /* winding number (0 without end game)              */
+  int32_t *eg_status;   /* PHC_ENDGAME_* (0 without end game)               */
+  double *eg_error;     /* endgame_result%error (0 without end game)        */
+  double *eg_residual;  /* optional: endgame_result%residual                */
+  int32_t *naccepted;   /* accepted steps, main path + end game             */
+  int32_t *nrejected;   /* rejected steps, main path + end game             */
+  int32_t *nlin;        /* optional: linearizations (evaluate + LU + solve) */
+  int32_t *nbe;         /* optional: backward-error evaluations             */
+} phc_endpoints;
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct phc_ctx phc_ctx;
+
+/* Creates a context on HIP device `device` (stream, device arenas, kernel
+ * configuration). Returns PHC_ERR_DEVICE if the device cannot be used. */
+int phc_ctx_create(int device, phc_ctx **ctx);
+int phc_ctx_destroy(phc_ctx *ctx);
+
+/* Lanes per path chosen for systems with N unknowns (4, 8 or 16), after
+ * autotuning if phc_autotune ran on this context. */
+int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
+
+/* Last error message of the context (static storage, never NULL). */
+const char *phc_ctx_last_error(const phc_ctx *ctx);
+
+/* ---- plain per-solve tracker ------------------------------------------ */
+/* Tracks paths first_path .. first_path+npaths-1 of one system (path k uses
+ * start point start[k - first_path]) and writes one endpoint record per
+ * path. `start` is [npaths][N][2] (host memory, tdh_start_solution,
+ * src/homotopy.f90:151-177). Synchronous. */
+int phc_track_paths(phc_ctx *ctx, const phc_system *sys,
+                    const phc_homotopy *hom, const phc_solve_options *opts,
+                    int64_t npaths, const double *start, phc_endpoints *out);
+
+/* Same, with the system, constants, start points and outputs already in
+ * device memory prepared by phc_stage_* below (used for timing with inputs
+ * resident in HBM). Asynchronous on the context stream; phc_ctx_sync waits. */
+typedef struct phc_staged phc_staged;
+int phc_stage_create(phc_ctx *ctx, const phc_system *sys,
+                     const phc_homotopy *hom, const phc_solve_options *opts,
+                     int64_t npaths, const double *start, phc_staged **st);
+int phc_stage_run(phc_ctx *ctx, phc_staged *st);
+int phc_stage_fetch(phc_ctx *ctx, phc_staged *st, phc_endpoints *out);
+int phc_stage_destroy(phc_ctx *ctx, phc_staged *st);
+int phc_ctx_sync(phc_ctx *ctx);
+
+/* ---- batched multi-system tracker ------------------------------------- */
+/* Tracks the paths of nsys independent solves in one persistent launch
+ * sequence; system s has npaths[s] paths with start points start[s] and
+ * writes out[s]. Equivalent, path by path, to nsys phc_track_paths calls. */
+int phc_track_batch(phc_ctx *ctx, int32_t nsys, const phc_system *sys,
+                    const phc_homotopy *hom, const phc_solve_options *opts,
+                    const int64_t *npaths, const double *const *start,
+                    phc_endpoints *out);
+
+/* ---- serving ----------------------------------------------------------- */
+typedef struct phc_server phc_server;
+typedef struct phc_server_config {
+  int64_t max_resident_paths;  /* path slots kept resident in HBM          */
+  int64_t hbm_budget_bytes;    /* admission budget for request state       */
+  int32_t units_per_launch;    /* linearizations per path per kernel launch */
+  int32_t reserved;
+} phc_server_config;
+
+int phc_server_create(phc_ctx *ctx, const phc_server_config *cfg,
+                      phc_server **srv);
+/* Queues one solve request; the inputs are copied before returning. The
+ * request is admitted to the device as soon as its state fits the budget. */
+int phc_server_submit(phc_server *srv, const phc_system *sys,
+                      const phc_homotopy *hom, const phc_solve_options *opts,
+                      int64_t npaths, const double *start, phc_endpoints *out,
+                      uint64_t *ticket);
+/* Advances the device work (admission, launches, retirement) once without
+ * blocking; returns the number of requests completed by this call. */
+int phc_server_progress(phc_server *srv);
+/* PHC_OK when the request's records are in its `out`, PHC_ERR_PENDING if
+ * not yet; phc_server_wait blocks (driving progress) until done. */
+int phc_server_poll(phc_server *srv, uint64_t ticket);
+int phc_server_wait(phc_server *srv, uint64_t ticket);
+int phc_server_destroy(phc_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHC_H */

@@ -1,0 +1,941 @@
+// phc_device.h -- the path tracker on the device.
+//
+// Mapping. One path is owned by a group of G lanes (G = 4, 8 or 16, a
+// power of two >= N = n + 1). Lane r of the group owns row r of every
+// N x N system and component r of every N-vector. Scalars of the path
+// state machine are replicated in all lanes of the group and updated
+// identically, so control flow is uniform inside a group. A wavefront holds
+// 64/G paths; a workgroup holds 256/G.
+//
+// Linearization unit (the only heavy work): at a point zq and parameter t
+//   1. lane j writes the powers zq_j^e, e = 1..maxexp, to the group's table
+//      in LDS (src/sparse.f90:251-255);
+//   2. lane i < n walks the terms of equation i and accumulates F^h_i and
+//      row i of dF^h/dz with the prefix/suffix products of
+//      src/sparse.f90:259-278, skipping factors that are exactly one;
+//   3. lane i assembles row i of H, H_z, H_t (src/homotopy.f90:488-511),
+//      lane n the patch row;
+//   4. LU with partial pivoting, row per lane; pivot search is a group
+//      max-reduction with the reference's first-maximum tie rule, rows are
+//      swapped logically (src/linalg.f90:26-78);
+//   5. forward/back substitution (src/linalg.f90:80-114) with the sums in
+//      the reference's order.
+// Arithmetic follows the reference operation by operation (see
+// phc_math.h), so device trajectories reproduce the reference bit for bit.
+//
+// State machine. `endpoint` (src/solve.f90:217-252) with track_segment
+// (src/tracker.f90:196-264), predict_rk4/tangent/newton_correct
+// (src/tracker.f90:266-360) and cauchy_endgame/loops/record
+// (src/endgame.f90:124-378) is flattened into a resumable state machine
+// whose only blocking points are linearization units (tangent, Newton
+// step, backward error). Between units a group runs cheap scalar logic;
+// every group of a wavefront then executes the next unit together.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "phc_math.h"
+#include "../../include/phc.h"
+
+namespace phc {
+
+#define PHC_D __device__ __forceinline__
+
+constexpr int TRIG_BITS = 12;
+constexpr int TRIG_M = 1 << TRIG_BITS;
+constexpr int WG = 256;  // threads per workgroup
+
+// One term of a packed system: coefficient and the exponent of unknown j in
+// byte j (j < 16).
+struct TermRec {
+  double re, im;
+  uint32_t e[4];
+};
+
+// One solve request: the packed system, its random constants, options and
+// output arrays (all device pointers).
+struct Req {
+  int32_t n, N, maxexp, nterms;
+  int32_t first[PHC_MAX_UNKNOWNS + 1];
+  int32_t deg[PHC_MAX_UNKNOWNS];
+  cx gamma;
+  cx b[PHC_MAX_UNKNOWNS];
+  cx patch[PHC_MAX_UNKNOWNS];
+  const TermRec* terms;
+  const cx* trig;   // [nsamples][TRIG_M + 1]: (cos, sin) of arc angles
+  const cx* start;  // [npaths][N]
+  int64_t npaths;
+  // tracker_options
+  double step_init, step_max, step_min, ctol, contraction;
+  int32_t maxit, grow_after, max_steps, use_endgame;
+  // endgame_options
+  double t_start, rf, rmin, loop_radius, agreement, egtol, rtol, closure;
+  int32_t nsamples, max_winding;
+  double dtheta, log_inv_rf;
+  // outputs
+  int32_t* o_status;
+  double* o_t;
+  cx* o_z;
+  int32_t* o_wind;
+  int32_t* o_egs;
+  double* o_egerr;
+  double* o_egres;
+  int32_t* o_acc;
+  int32_t* o_rej;
+  int32_t* o_nlin;
+  int32_t* o_nbe;
+  uint32_t* o_done;  // paths of this request finished
+};
+
+// What a group waits for.
+enum : int32_t { PC_IDLE = 0, PC_TAN = 1, PC_NEWT = 2, PC_BE = 3 };
+// Where track_segment returns to.
+enum : int32_t { RET_MAIN = 0, RET_RADIAL = 1, RET_ARC = 2 };
+// Caller of loops / record inside cauchy_endgame.
+enum : int32_t { CTX_MAIN = 0, CTX_FALLBACK = 1 };
+
+struct PathState {
+  int32_t pc, req, path, ret;
+  // track_segment (src/tracker.f90:207-221)
+  int32_t sk, sj, stage, nit, nsucc, sacc, srej, sstatus, o_grow;
+  double sa, sb, sdth, s, ds, sn, step, se, dnp, o_step_max;
+  // endpoint (src/solve.f90:226-251)
+  int32_t pstatus, nacc, nrej, nlin, nbe;
+  double pt;
+  // cauchy_endgame (src/endgame.f90:135-174)
+  int32_t have_prev, gate, max_loops, eg_status, eg_wind, eg_acc, eg_rej;
+  int32_t lc, lj, lmax, lctx, nsamp, rctx, lok;
+  double r, rnew, best_resid, dprev, cest, cest_prev, eg_err, eg_resid,
+      seg_init, rec_err, spacing, floor;
+  // this lane's component of the path's vectors
+  cx zt;    // point tracked by the current segment
+  cx zq;    // evaluation point / Newton iterate
+  cx acc;   // RK4 accumulator k1 + 2 k2 + 2 k3 + k4
+  cx zeg;   // end game point on the real axis
+  cx est, prev, zold, resz, zprev, zsum;
+};
+
+struct Launch {
+  const Req* reqs;
+  const int64_t* req_base;  // [nreq + 1] prefix sums of path counts
+  int32_t nreq;
+  int32_t expmax;           // largest maxexp of the requests
+  unsigned long long* next; // work-queue head
+  int64_t total;            // paths in the queue
+  PathState* slots;         // [groups][G] saved state (resumable launches)
+  int32_t resume;           // load saved states at entry
+  int32_t max_units;        // units per group per launch, 0 = unlimited
+  unsigned int* active;     // groups still holding a path at exit
+  unsigned int* counters;   // [4]: trig fallbacks, units, be units, paths
+};
+
+// ---------------------------------------------------------------- group ops
+template <int G>
+PHC_D double gshfl(double v, int src) {
+  return __shfl(v, src, G);
+}
+template <int G>
+PHC_D int gshfl_i(int v, int src) {
+  return __shfl(v, src, G);
+}
+template <int G>
+PHC_D cx gshfl_c(cx v, int src) {
+  return mk(__shfl(v.re, src, G), __shfl(v.im, src, G));
+}
+template <int G>
+PHC_D double gmax(double v) {
+#pragma unroll
+  for (int m = 1; m < G; m <<= 1) v = fmax(v, __shfl_xor(v, m, G));
+  return v;
+}
+// max_i |v_i| over the N components held by the group (src/linalg.f90:176).
+template <int G>
+PHC_D double gnorm(cx v, int lane, int N) {
+  return gmax<G>(lane < N ? cabs_(v) : 0.0);
+}
+
+PHC_D void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------------ segment
+// seg_t / seg_dt (src/tracker.f90:142-177). Lines are real (all lines the
+// tracker uses are), arcs look their angle up in the host-computed table
+// when s is a multiple of 2^-TRIG_BITS.
+PHC_D cx arc_cs(const Req& R, const PathState& S, double s,
+                unsigned int* fallbacks) {
+  double m = s * (double)TRIG_M;
+  if (m >= 0.0 && m <= (double)TRIG_M && m == rint(m)) {
+    const cx* tab = R.trig + (size_t)S.sj * (TRIG_M + 1);
+    return tab[(int)m];
+  }
+  double theta = S.sa + s * S.sdth;
+  if (fallbacks) atomicAdd(fallbacks, 1u);
+  return mk(cos(theta), sin(theta));
+}
+
+PHC_D cx seg_t(const Req& R, const PathState& S, double s,
+               unsigned int* fb) {
+  if (S.sk == 0) {
+    if (s >= 1.0) return mk(S.sb, 0.0);
+    return mk(S.sa + s * (S.sb - S.sa), 0.0);
+  }
+  cx cs = arc_cs(R, S, s, fb);
+  return cscale(S.sb, cs);  // radius in sb
+}
+
+PHC_D cx seg_dt(const Req& R, const PathState& S, double s,
+                unsigned int* fb) {
+  if (S.sk == 0) return mk(S.sb - S.sa, 0.0);
+  cx t = seg_t(R, S, s, fb);
+  return mk(0.0 * t.re - S.sdth * t.im, 0.0 * t.im + S.sdth * t.re);
+}
+
+// -------------------------------------------------------------------- unit
+// Evaluates H, H_z, H_t at (zq, t) and, for PC_TAN / PC_NEWT, solves
+// H_z x = -(H_t dt) / -H. Returns x_r in lane r. For PC_BE returns the
+// backward error of src/endgame.f90:269-285 in *be.
+template <int G>
+PHC_D void unit(const Req& R, const int lane, const int mode, const cx zq,
+                const cx t, const cx dt, cx* __restrict__ T, cx& xout,
+                int& okout, double& be) {
+  const int N = R.N, n = R.n;
+  const bool row = lane < N;
+
+  // 1. power table pw(e, j) = zq_j^e, e = 1..maxexp (src/sparse.f90:252-255)
+  wave_lds_fence();
+  if (row) {
+    cx p = zq;
+    T[G + lane] = p;
+    for (int e = 2; e <= R.maxexp; ++e) {
+      p = cmul(p, zq);
+      T[e * G + lane] = p;
+    }
+  }
+  wave_lds_fence();
+
+  // 2. F^h_i and row i of the Jacobian (src/sparse.f90:257-278)
+  cx f = mk(0.0, 0.0);
+  cx hz[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) hz[j] = mk(0.0, 0.0);
+  int k0 = 0, k1 = 0;
+  if (lane < n) {
+    k0 = R.first[lane];
+    k1 = R.first[lane + 1];
+  }
+  for (int k = k0; k < k1; ++k) {
+    const TermRec tr = R.terms[k];
+    const cx c = mk(tr.re, tr.im);
+    int ex[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) ex[j] = (tr.e[j >> 2] >> (8 * (j & 3))) & 255;
+    // suffix products over the nonzero exponents
+    cx suf[G];
+    bool sfs[G];
+    cx S = mk(1.0, 0.0);
+    bool sset = false;
+#pragma unroll
+    for (int j = G - 1; j >= 0; --j) {
+      sfs[j] = sset;
+      suf[j] = S;
+      if (ex[j] > 0) {
+        const cx pv = T[ex[j] * G + j];
+        S = sset ? cmul(pv, S) : pv;
+        sset = true;
+      }
+    }
+    // prefix products, derivatives, value
+    cx P = mk(1.0, 0.0);
+    bool pset = false;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      if (ex[j] > 0) {
+        cx d = cscale((double)ex[j], c);
+        if (ex[j] >= 2) d = cmul(d, T[(ex[j] - 1) * G + j]);
+        if (pset) d = cmul(d, P);
+        if (sfs[j]) d = cmul(d, suf[j]);
+        hz[j] = cadd(hz[j], d);
+        const cx pv = T[ex[j] * G + j];
+        P = pset ? cmul(P, pv) : pv;
+        pset = true;
+      }
+    }
+    f = cadd(f, cmul(c, P));
+  }
+
+  // 3. homotopy rows (src/homotopy.f90:491-511)
+  const cx w = gshfl_c<G>(zq, n);
+  cx h = mk(0.0, 0.0), ht = mk(0.0, 0.0);
+  if (lane < n) {
+    const int d = R.deg[lane];
+    const cx gt = cmul(R.gamma, t);
+    const cx omt = mk(1.0 - t.re, 0.0 - t.im);
+#pragma unroll
+    for (int j = 0; j < G; ++j) hz[j] = cmul(omt, hz[j]);
+    const cx zd1 = zpowi(zq, d - 1);
+    const cx wd1 = zpowi(w, d - 1);
+    const cx bi = R.b[lane];
+    const cx g = csub(cmul(zd1, zq), cmul(cmul(bi, wd1), w));
+    h = cadd(cmul(gt, g), cmul(omt, f));
+    ht = csub(cmul(R.gamma, g), f);
+    const cx dg = cmul(cscale((double)d, gt), zd1);
+    const cx dw = cmul(cscale((double)d, cmul(gt, bi)), wd1);
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      if (j == lane) hz[j] = cadd(hz[j], dg);
+      if (j == n) hz[j] = csub(hz[j], dw);
+    }
+  }
+  // patch row: h_{n+1} = sum(a*z) - 1 with the sum in index order
+  {
+    const cx pz = row ? cmul(R.patch[lane], zq) : mk(0.0, 0.0);
+    cx ssum = mk(0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const cx pj = gshfl_c<G>(pz, j);
+      if (j < N) ssum = cadd(ssum, pj);
+    }
+    if (lane == n) {
+      h = mk(ssum.re - 1.0, ssum.im - 0.0);
+#pragma unroll
+      for (int j = 0; j < G; ++j) hz[j] = (j < N) ? R.patch[j] : mk(0.0, 0.0);
+    }
+  }
+
+  if (mode == PC_BE) {
+    // ||H(z,0)|| / max(max_i sum_j |H_z(i,j)| * ||z||, tiny)
+    const double hn = gmax<G>(row ? cabs_(h) : 0.0);
+    double rs = 0.0;
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (j < N) rs = rs + cabs_(hz[j]);
+    const double rmax = gmax<G>(row ? rs : 0.0);
+    const double zn = gnorm<G>(zq, lane, N);
+    be = hn / fmax(rmax * zn, DBL_MIN);
+    okout = 1;
+    xout = mk(0.0, 0.0);
+    return;
+  }
+
+  // right-hand side: tangent -(H_t dt/ds), Newton -H (src/tracker.f90:314,348)
+  const cx b = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
+
+  // 4. LU with partial pivoting, logical row swaps (src/linalg.f90:49-76)
+  int pos = row ? lane : 1 << 20;
+  int pl[G];
+  int ok = 1;
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    pl[k] = 0;
+    if (k < N && ok) {
+      double bv = (row && pos >= k) ? cabs_(hz[k]) : -1.0;
+      int bp = pos, bl = lane;
+#pragma unroll
+      for (int m = 1; m < G; m <<= 1) {
+        const double ov = __shfl_xor(bv, m, G);
+        const int op = __shfl_xor(bp, m, G);
+        const int ol = __shfl_xor(bl, m, G);
+        if (ov > bv || (ov == bv && op < bp)) {
+          bv = ov;
+          bp = op;
+          bl = ol;
+        }
+      }
+      if (bv == 0.0) {
+        ok = 0;
+      } else {
+        pl[k] = bl;
+        int np = pos;
+        if (pos == k) np = bp;
+        if (lane == bl) np = k;
+        pos = np;
+        const cx piv = gshfl_c<G>(hz[k], bl);
+        const bool below = row && pos > k;
+        cx l = hz[k];
+        if (below) {
+          l = cdiv(hz[k], piv);
+          hz[k] = l;
+        }
+#pragma unroll
+        for (int j = k + 1; j < G; ++j) {
+          if (j < N) {
+            const cx pj = gshfl_c<G>(hz[j], bl);
+            if (below) hz[j] = csub(hz[j], cmul(l, pj));
+          }
+        }
+      }
+    }
+  }
+  okout = ok;
+  if (!ok) {
+    xout = mk(0.0, 0.0);
+    return;
+  }
+
+  // 5. forward substitution with unit L in logical row order
+  cx sacc = mk(0.0, 0.0);
+  bool sset = false;
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    if (k < N) {
+      const cx cand = (pos == 0 || !sset) ? b : csub(b, sacc);
+      const cx yk = gshfl_c<G>(cand, pl[k]);
+      if (row && pos > k) {
+        const cx p = cmul(hz[k], yk);
+        sacc = sset ? cadd(sacc, p) : p;
+        sset = true;
+      }
+    }
+  }
+  const cx yfin = (pos == 0 || !sset) ? b : csub(b, sacc);
+  // back substitution, sums in increasing column order
+  cx xs[G];
+  cx xmine = mk(0.0, 0.0);
+#pragma unroll
+  for (int i = G - 1; i >= 0; --i) {
+    xs[i] = mk(0.0, 0.0);
+    if (i < N) {
+      cx s = mk(0.0, 0.0);
+      bool ss = false;
+#pragma unroll
+      for (int j = i + 1; j < G; ++j) {
+        if (j < N) {
+          const cx p = cmul(hz[j], xs[j]);
+          s = ss ? cadd(s, p) : p;
+          ss = true;
+        }
+      }
+      const cx num = ss ? csub(yfin, s) : yfin;
+      const cx cand = cdiv(num, hz[i]);
+      const cx xi = gshfl_c<G>(cand, pl[i]);
+      xs[i] = xi;
+      if (lane == i) xmine = xi;
+    }
+  }
+  xout = xmine;
+}
+
+// ------------------------------------------------------------ state machine
+// Internal states: a transition never calls another transition, it only
+// sets S.pc; Machine::run() dispatches until the group needs a unit
+// (PC_TAN, PC_NEWT, PC_BE) or the path is done (PC_IDLE).
+enum : int32_t {
+  X_SEG_HEAD = 16,
+  X_SEG_FINISH,
+  X_MAIN_DONE,
+  X_EG_TOP,
+  X_EG_AFTER_LOOPS_MAIN,
+  X_EG_AFTER_RECORD_MAIN,
+  X_EG_RADIAL,
+  X_EG_AFTER_RADIAL,
+  X_EG_AFTER_LOOPS_FALLBACK,
+  X_EG_RETURN,
+  X_LOOPS_BEGIN,
+  X_LOOPS_ARC,
+  X_LOOPS_AFTER_ARC,
+  X_LOOPS_END,
+  X_RECORD_BEGIN,
+};
+
+// All functions below run with group-uniform control flow.
+template <int G>
+struct Machine {
+  const Req& R;
+  PathState& S;
+  const int lane;
+  unsigned int* fb;
+
+  PHC_D double norm(cx v) const { return gnorm<G>(v, lane, R.N); }
+
+  // --- track_segment -------------------------------------------------------
+  PHC_D void seg_begin(int kind, double a, double b, int j0, int ret,
+                       double step_init, double step_max, int grow) {
+    S.sk = kind;
+    S.sj = j0;
+    S.sa = a;  // line: t0; arc: theta0 = (j - 1) dtheta
+    S.sb = b;  // line: t1; arc: radius
+    S.sdth = kind ? R.dtheta : 0.0;
+    S.ret = ret;
+    S.s = 0.0;
+    S.step = fmin(step_init, step_max);
+    S.o_step_max = step_max;
+    S.o_grow = grow;
+    S.nsucc = 0;
+    S.sacc = 0;
+    S.srej = 0;
+    S.sstatus = PHC_PATH_SUCCESS;
+    S.pc = X_SEG_HEAD;
+  }
+
+  // Loop head of track_segment (src/tracker.f90:222-236).
+  PHC_D void seg_head() {
+    if (!(S.s < 1.0)) {
+      S.pc = X_SEG_FINISH;
+      return;
+    }
+    if (S.sacc + S.srej >= R.max_steps) {
+      S.sstatus = PHC_PATH_FAILED_MAX_STEPS;
+      S.pc = X_SEG_FINISH;
+      return;
+    }
+    S.ds = fmin(S.step, 1.0 - S.s);
+    S.sn = S.s + S.ds;
+    if (1.0 - S.sn <= 1.0e-3 * S.step) {
+      S.sn = 1.0;
+      S.ds = 1.0 - S.s;
+    }
+    S.stage = 0;
+    S.zq = S.zt;
+    S.se = S.s;
+    S.pc = PC_TAN;
+  }
+
+  // src/tracker.f90:240-257
+  PHC_D void seg_accept() {
+    S.zt = S.zq;
+    S.s = S.sn;
+    S.sacc += 1;
+    S.nsucc += 1;
+    if (S.nsucc >= S.o_grow) {
+      S.step = fmin(2.0 * S.step, S.o_step_max);
+      S.nsucc = 0;
+    }
+    S.pc = X_SEG_HEAD;
+  }
+
+  PHC_D void seg_reject() {
+    S.srej += 1;
+    S.nsucc = 0;
+    S.step = 0.5 * S.step;
+    if (S.step < R.step_min) {
+      S.sstatus = PHC_PATH_FAILED_MIN_STEP;
+      S.pc = X_SEG_FINISH;
+      return;
+    }
+    S.pc = X_SEG_HEAD;
+  }
+
+  // predict_rk4 (src/tracker.f90:283-291)
+  PHC_D void on_tangent(cx k, int ok) {
+    S.nlin += 1;
+    if (!ok) {
+      seg_reject();
+      return;
+    }
+    if (S.stage == 0) {
+      S.acc = k;
+      S.zq = cadd(S.zt, cscale(0.5 * S.ds, k));
+      S.se = S.s + 0.5 * S.ds;
+      S.stage = 1;
+      S.pc = PC_TAN;
+    } else if (S.stage == 1) {
+      S.acc = cadd(S.acc, cscale(2.0, k));
+      S.zq = cadd(S.zt, cscale(0.5 * S.ds, k));
+      S.se = S.s + 0.5 * S.ds;
+      S.stage = 2;
+      S.pc = PC_TAN;
+    } else if (S.stage == 2) {
+      S.acc = cadd(S.acc, cscale(2.0, k));
+      S.zq = cadd(S.zt, cscale(S.ds, k));
+      S.se = S.s + S.ds;
+      S.stage = 3;
+      S.pc = PC_TAN;
+    } else {
+      S.acc = cadd(S.acc, k);
+      S.zq = cadd(S.zt, cscale(S.ds / 6.0, S.acc));
+      S.nit = 1;
+      S.dnp = DBL_MAX;
+      if (R.maxit < 1)
+        seg_reject();
+      else
+        S.pc = PC_NEWT;
+    }
+  }
+
+  // newton_correct (src/tracker.f90:342-358)
+  PHC_D void on_newton(cx dz, int ok) {
+    S.nlin += 1;
+    if (!ok) {
+      seg_reject();
+      return;
+    }
+    S.zq = cadd(S.zq, dz);
+    const double dn = norm(dz);
+    const double zn = norm(S.zq);
+    if (dn <= R.ctol * zn) {
+      seg_accept();
+      return;
+    }
+    if (S.nit > 1 && dn > R.contraction * S.dnp) {
+      seg_reject();
+      return;
+    }
+    S.dnp = dn;
+    if (S.nit >= R.maxit) {
+      seg_reject();
+      return;
+    }
+    S.nit += 1;
+    S.pc = PC_NEWT;
+  }
+
+  PHC_D void seg_finish() {
+    S.pc = (S.ret == RET_MAIN)     ? X_MAIN_DONE
+           : (S.ret == RET_RADIAL) ? X_EG_AFTER_RADIAL
+                                   : X_LOOPS_AFTER_ARC;
+  }
+
+  // --- endpoint (src/solve.f90:230-251) -------------------------------------
+  PHC_D void start_path(const cx zstart) {
+    S.zt = zstart;
+    S.nacc = S.nrej = S.nlin = S.nbe = 0;
+    const double t_end = R.use_endgame ? R.t_start : 0.0;
+    seg_begin(0, 1.0, t_end, 0, RET_MAIN, R.step_init, R.step_max,
+              R.grow_after);
+  }
+
+  PHC_D void main_done() {
+    S.pstatus = S.sstatus;
+    S.pt = seg_t(R, S, S.s, fb).re;
+    S.nacc = S.sacc;
+    S.nrej = S.srej;
+    if (S.sstatus != PHC_PATH_SUCCESS || !R.use_endgame) {
+      finish(S.zt, 0, 0, 0.0, 0.0);
+      return;
+    }
+    // cauchy_endgame set-up (src/endgame.f90:160-174)
+    S.seg_init = 1.0;  // segopts: step_init = step_max = 1, grow_after = 2
+    S.eg_status = PHC_ENDGAME_FAILED;
+    S.resz = S.zt;
+    S.zeg = S.zt;
+    S.r = R.t_start;
+    S.have_prev = 0;
+    S.best_resid = DBL_MAX;
+    S.gate = 0;
+    S.dprev = -1.0;
+    S.cest = -1.0;
+    S.cest_prev = -1.0;
+    S.max_loops = R.max_winding;
+    S.eg_wind = 0;
+    S.eg_err = DBL_MAX;
+    S.eg_resid = DBL_MAX;
+    S.eg_acc = 0;
+    S.eg_rej = 0;
+    S.pc = X_EG_TOP;
+  }
+
+  // --- cauchy_endgame main loop (src/endgame.f90:176-236) --------------------
+  PHC_D void eg_top() {
+    if (S.gate || S.r < R.loop_radius) {
+      if (!S.gate) S.max_loops = R.max_winding;
+      S.lmax = S.max_loops;
+      S.lctx = CTX_MAIN;
+      S.pc = X_LOOPS_BEGIN;
+      return;
+    }
+    S.lok = 0;
+    S.pc = X_EG_AFTER_LOOPS_MAIN;
+  }
+
+  PHC_D void eg_after_loops_main() {
+    if (!S.lok) {
+      S.have_prev = 0;
+      S.gate = 0;
+      S.pc = X_EG_RADIAL;
+      return;
+    }
+    S.rctx = CTX_MAIN;
+    S.pc = X_RECORD_BEGIN;
+  }
+
+  PHC_D void eg_after_record_main() {
+    if (S.eg_status == PHC_ENDGAME_CONVERGED) {
+      S.pc = X_EG_RETURN;
+      return;
+    }
+    S.prev = S.est;
+    S.have_prev = 1;
+    S.pc = X_EG_RADIAL;
+  }
+
+  PHC_D void eg_radial() {
+    S.rnew = R.rf * S.r;
+    if (S.rnew < R.rmin) {
+      S.pc = X_EG_RETURN;
+      return;
+    }
+    S.zold = S.zeg;
+    S.zt = S.zeg;
+    seg_begin(0, S.r, S.rnew, 0, RET_RADIAL, S.seg_init, 1.0, 2);
+  }
+
+  PHC_D void eg_after_radial() {
+    S.zeg = S.zt;
+    S.seg_init = fmin(fmax(S.step, R.step_min), 1.0);
+    S.eg_acc += S.sacc;
+    S.eg_rej += S.srej;
+    if (S.sstatus != PHC_PATH_SUCCESS) {
+      S.zeg = S.zold;
+      if (!(S.gate || S.r < R.loop_radius)) {
+        S.lmax = R.max_winding;
+        S.lctx = CTX_FALLBACK;
+        S.pc = X_LOOPS_BEGIN;
+        return;
+      }
+      S.pc = X_EG_RETURN;
+      return;
+    }
+    S.r = S.rnew;
+    const double d = norm(csub(S.zeg, S.zold));
+    if (!S.gate && S.dprev > 0.0 && d > 0.0) {
+      S.cest_prev = S.cest;
+      S.cest = -1.0;
+      if (S.dprev > d) S.cest = R.log_inv_rf / log(S.dprev / d);
+      if (S.cest >= 0.5 && S.cest <= (double)R.max_winding + 0.5 &&
+          S.cest_prev > 0.0) {
+        if (fabs(S.cest - S.cest_prev) <= R.agreement * fmax(1.0, S.cest)) {
+          S.gate = 1;
+          S.max_loops = min(R.max_winding, 2 * (int)round(S.cest) + 2);
+        }
+      }
+    }
+    S.dprev = d;
+    S.pc = X_EG_TOP;
+  }
+
+  PHC_D void eg_after_loops_fallback() {
+    S.have_prev = 0;
+    if (S.lok) {
+      S.rctx = CTX_FALLBACK;
+      S.pc = X_RECORD_BEGIN;
+      return;
+    }
+    S.pc = X_EG_RETURN;
+  }
+
+  // record (src/endgame.f90:240-265)
+  PHC_D void record_begin() {
+    S.rec_err = DBL_MAX;
+    if (S.have_prev)
+      S.rec_err = norm(csub(S.est, S.prev)) / fmax(norm(S.est), DBL_MIN);
+    S.zq = S.est;
+    S.pc = PC_BE;
+  }
+
+  PHC_D void on_backward_error(double resid) {
+    S.nbe += 1;
+    const double err = S.rec_err;
+    bool take = S.eg_status == PHC_ENDGAME_FAILED;
+    if (!take) {  // better (src/endgame.f90:287-311)
+      const bool valid = resid <= R.rtol;
+      const bool best_valid = S.best_resid <= R.rtol;
+      take = (valid != best_valid) ? valid : (err < S.eg_err);
+    }
+    if (take) {
+      S.eg_status = PHC_ENDGAME_NOT_CONVERGED;
+      S.resz = S.est;
+      S.eg_wind = S.lc;
+      S.eg_err = err;
+      S.eg_resid = resid;
+      S.best_resid = resid;
+    }
+    if (err <= R.egtol && resid <= R.rtol) S.eg_status = PHC_ENDGAME_CONVERGED;
+    S.pc = (S.rctx == CTX_MAIN) ? X_EG_AFTER_RECORD_MAIN : X_EG_RETURN;
+  }
+
+  PHC_D void eg_return() {
+    S.nacc += S.eg_acc;
+    S.nrej += S.eg_rej;
+    if (S.eg_status == PHC_ENDGAME_FAILED) {
+      S.pstatus = PHC_PATH_ENDGAME_FAILED;
+      finish(S.zeg, 0, 0, 0.0, 0.0);
+      return;
+    }
+    S.pt = 0.0;
+    finish(S.resz, S.eg_wind, S.eg_status, S.eg_err, S.eg_resid);
+  }
+
+  // loops (src/endgame.f90:349-376)
+  PHC_D void loops_begin() {
+    S.floor = 10.0 * R.ctol * norm(S.zeg);
+    S.zt = S.zeg;
+    S.zsum = mk(0.0, 0.0);
+    S.nsamp = 0;
+    S.spacing = DBL_MAX;
+    S.lc = 1;
+    S.lj = 1;
+    if (S.lc > S.lmax) {
+      S.lok = 0;
+      S.pc = X_LOOPS_END;
+      return;
+    }
+    S.pc = X_LOOPS_ARC;
+  }
+
+  PHC_D void loops_arc() {
+    S.zsum = cadd(S.zsum, S.zt);
+    S.nsamp += 1;
+    S.zprev = S.zt;
+    seg_begin(1, (double)(S.lj - 1) * R.dtheta, S.r, S.lj - 1, RET_ARC,
+              S.seg_init, 1.0, 2);
+  }
+
+  PHC_D void loops_after_arc() {
+    S.seg_init = fmin(fmax(S.step, R.step_min), 1.0);
+    S.eg_acc += S.sacc;
+    S.eg_rej += S.srej;
+    if (S.sstatus != PHC_PATH_SUCCESS) {
+      S.lok = 0;
+      S.pc = X_LOOPS_END;
+      return;
+    }
+    S.spacing = fmin(S.spacing, norm(csub(S.zt, S.zprev)));
+    S.lj += 1;
+    if (S.lj <= R.nsamples) {
+      S.pc = X_LOOPS_ARC;
+      return;
+    }
+    if (norm(csub(S.zt, S.zeg)) <= fmax(R.closure * S.spacing, S.floor)) {
+      S.est = cdiv(S.zsum, mk((double)S.nsamp, 0.0));
+      S.lok = 1;
+      S.pc = X_LOOPS_END;
+      return;
+    }
+    S.lc += 1;
+    S.lj = 1;
+    if (S.lc > S.lmax) {
+      S.lok = 0;
+      S.pc = X_LOOPS_END;
+      return;
+    }
+    S.pc = X_LOOPS_ARC;
+  }
+
+  // --- results -------------------------------------------------------------
+  PHC_D void finish(cx z, int wind, int egs, double egerr, double egres) {
+    const int64_t k = S.path;
+    if (lane < R.N) R.o_z[k * R.N + lane] = z;
+    if (lane == 0) {
+      R.o_status[k] = S.pstatus;
+      R.o_t[k] = S.pt;
+      R.o_wind[k] = wind;
+      R.o_egs[k] = egs;
+      R.o_egerr[k] = egerr;
+      if (R.o_egres) R.o_egres[k] = egres;
+      R.o_acc[k] = S.nacc;
+      R.o_rej[k] = S.nrej;
+      if (R.o_nlin) R.o_nlin[k] = S.nlin;
+      if (R.o_nbe) R.o_nbe[k] = S.nbe;
+      __threadfence();
+      if (R.o_done) atomicAdd(R.o_done, 1u);
+    }
+    S.pc = PC_IDLE;
+  }
+
+  // Dispatch until a unit is needed or the path is finished.
+  PHC_D void run() {
+    for (;;) {
+      switch (S.pc) {
+        case X_SEG_HEAD: seg_head(); break;
+        case X_SEG_FINISH: seg_finish(); break;
+        case X_MAIN_DONE: main_done(); break;
+        case X_EG_TOP: eg_top(); break;
+        case X_EG_AFTER_LOOPS_MAIN: eg_after_loops_main(); break;
+        case X_EG_AFTER_RECORD_MAIN: eg_after_record_main(); break;
+        case X_EG_RADIAL: eg_radial(); break;
+        case X_EG_AFTER_RADIAL: eg_after_radial(); break;
+        case X_EG_AFTER_LOOPS_FALLBACK: eg_after_loops_fallback(); break;
+        case X_EG_RETURN: eg_return(); break;
+        case X_LOOPS_BEGIN: loops_begin(); break;
+        case X_LOOPS_ARC: loops_arc(); break;
+        case X_LOOPS_AFTER_ARC: loops_after_arc(); break;
+        case X_LOOPS_END:
+          S.pc = (S.lctx == CTX_MAIN) ? X_EG_AFTER_LOOPS_MAIN
+                                      : X_EG_AFTER_LOOPS_FALLBACK;
+          break;
+        case X_RECORD_BEGIN: record_begin(); break;
+        default: return;  // PC_IDLE, PC_TAN, PC_NEWT, PC_BE
+      }
+    }
+  }
+};
+
+// --------------------------------------------------------------------- kernel
+template <int G>
+__global__ void __launch_bounds__(WG)
+    track_kernel(Launch L) {
+  extern __shared__ cx lds[];
+  const int lane = threadIdx.x % G;
+  const int gi = threadIdx.x / G;
+  const int slot = blockIdx.x * (WG / G) + gi;
+  cx* T = lds + (size_t)gi * (L.expmax + 1) * G;
+
+  PathState S;
+  if (L.resume) {
+    S = L.slots[(size_t)slot * G + lane];
+  } else {
+    S.pc = PC_IDLE;
+  }
+  int units = 0;
+  unsigned int* fb = L.counters;
+  for (;;) {
+    if (S.pc == PC_IDLE) {
+      unsigned long long idx = 0;
+      if (lane == 0) idx = atomicAdd(L.next, 1ull);
+      idx = __shfl((long long)idx, 0, G);
+      if ((long long)idx >= L.total) break;
+      int q = 0;
+      int lo = 0, hi = L.nreq - 1;
+      while (lo < hi) {  // last request with base <= idx
+        const int mid = (lo + hi + 1) >> 1;
+        if (L.req_base[mid] <= (long long)idx)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      q = lo;
+      S.req = q;
+      S.path = (int32_t)((long long)idx - L.req_base[q]);
+      const Req& R = L.reqs[q];
+      Machine<G> M{R, S, lane, fb};
+      const cx z0 = (lane < R.N) ? R.start[(size_t)S.path * R.N + lane]
+                                 : mk(0.0, 0.0);
+      M.start_path(z0);
+      M.run();
+      if (S.pc == PC_IDLE) continue;
+    }
+    if (L.max_units && units >= L.max_units) break;
+    const Req& R = L.reqs[S.req];
+    Machine<G> M{R, S, lane, fb};
+    cx t = mk(0.0, 0.0), dt = mk(0.0, 0.0);
+    if (S.pc == PC_TAN) {
+      t = seg_t(R, S, S.se, fb);
+      dt = seg_dt(R, S, S.se, fb);
+    } else if (S.pc == PC_NEWT) {
+      t = seg_t(R, S, S.sn, fb);
+    }
+    cx x;
+    int ok;
+    double be = 0.0;
+    unit<G>(R, lane, S.pc, (lane < R.N) ? S.zq : mk(0.0, 0.0), t, dt, T, x,
+            ok, be);
+    units++;
+    if (S.pc == PC_TAN)
+      M.on_tangent(x, ok);
+    else if (S.pc == PC_NEWT)
+      M.on_newton(x, ok);
+    else
+      M.on_backward_error(be);
+    M.run();
+  }
+  if (L.slots) L.slots[(size_t)slot * G + lane] = S;
+  if (S.pc != PC_IDLE && lane == 0 && L.active) atomicAdd(L.active, 1u);
+  if (lane == 0 && L.counters) atomicAdd(&L.counters[1], (unsigned)units);
+}
+
+}  // namespace phc

@@ -1,0 +1,581 @@
+// phc_runtime.hip -- host side of libphc_hip.so: the C ABI of include/phc.h.
+//
+// A call packs each solve request (system, random constants, options) into
+// a device `Req` record, uploads start points, and runs the persistent
+// tracking kernel over a device work queue of (request, path) items until
+// every path is done. Requests are grouped by lanes-per-path class (G = 4,
+// 8, 16 from N = n + 1); each class is one kernel instantiation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/phc.h"
+#include "phc_device.h"
+
+using namespace phc;
+
+struct phc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  int lanes_override[PHC_MAX_UNKNOWNS + 1] = {0};
+  int wg_threads = 0;  // 0: choose per launch
+  std::map<int, cx*> trig;  // nsamples -> device table
+  std::mutex mu;
+  char err[512] = {0};
+};
+
+namespace {
+
+int fail(phc_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                  \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return fail(ctx, PHC_ERR_DEVICE, "%s failed: %s", #call,             \
+                  hipGetErrorString(e_));                                  \
+  } while (0)
+
+int lanes_for(const phc_ctx* ctx, int N) {
+  if (ctx && N >= 0 && N <= PHC_MAX_UNKNOWNS && ctx->lanes_override[N])
+    return ctx->lanes_override[N];
+  if (N <= 4) return 4;
+  if (N <= 8) return 8;
+  return 16;
+}
+
+// Device arena: one hipMalloc per call, bump-allocated.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  ~Arena() {
+    if (base) (void)hipFree(base);
+  }
+  static size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+};
+
+struct HostReq {
+  Req r;                         // device pointers filled at upload
+  std::vector<TermRec> terms;
+  const double* start = nullptr; // host [npaths][N][2]
+  phc_endpoints* out = nullptr;
+  int64_t npaths = 0;
+  int G = 8;
+};
+
+int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
+                      const phc_homotopy* hom, const phc_solve_options* o,
+                      int64_t npaths, HostReq& h) {
+  if (!sys || !hom || !o) return fail(ctx, PHC_ERR_INVALID_INPUT, "null input");
+  const int n = sys->n, N = n + 1;
+  if (n < 1 || N > PHC_MAX_UNKNOWNS)
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "n = %d outside 1..%d", n,
+                PHC_MAX_UNKNOWNS - 1);
+  if (sys->nterms < n || !sys->first || !sys->deg || !sys->hexps ||
+      !sys->coefs)
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "incomplete system");
+  if (npaths < 0) return fail(ctx, PHC_ERR_INVALID_INPUT, "npaths < 0");
+  Req& r = h.r;
+  memset(&r, 0, sizeof(r));
+  r.n = n;
+  r.N = N;
+  r.nterms = sys->nterms;
+  if (sys->first[0] != 0 || sys->first[n] != sys->nterms)
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "bad term offsets");
+  int maxexp = 0;
+  for (int i = 0; i <= n; ++i) r.first[i] = sys->first[i];
+  for (int i = 0; i < n; ++i) {
+    if (sys->first[i + 1] <= sys->first[i])
+      return fail(ctx, PHC_ERR_INVALID_INPUT, "equation %d has no terms", i);
+    if (sys->deg[i] < 1)
+      return fail(ctx, PHC_ERR_INVALID_INPUT, "degree of equation %d < 1", i);
+    r.deg[i] = sys->deg[i];
+  }
+  h.terms.resize(sys->nterms);
+  for (int k = 0; k < sys->nterms; ++k) {
+    TermRec& t = h.terms[k];
+    memset(&t, 0, sizeof(t));
+    t.re = sys->coefs[2 * k];
+    t.im = sys->coefs[2 * k + 1];
+    int tot = 0;
+    for (int j = 0; j < N; ++j) {
+      const int e = sys->hexps[(size_t)k * N + j];
+      if (e < 0 || e > 255)
+        return fail(ctx, PHC_ERR_INVALID_INPUT,
+                    "exponent %d of term %d outside 0..255", e, k);
+      t.e[j >> 2] |= (uint32_t)e << (8 * (j & 3));
+      maxexp = std::max(maxexp, e);
+      tot += e;
+    }
+    (void)tot;
+  }
+  r.maxexp = std::max(maxexp, 1);
+  r.gamma = mk(hom->gamma[0], hom->gamma[1]);
+  for (int i = 0; i < n; ++i) r.b[i] = mk(hom->b[2 * i], hom->b[2 * i + 1]);
+  for (int j = 0; j < N; ++j)
+    r.patch[j] = mk(hom->patch[2 * j], hom->patch[2 * j + 1]);
+  const phc_tracker_options& t = o->tracker;
+  const phc_endgame_options& e = o->endgame;
+  r.step_init = t.step_init;
+  r.step_max = t.step_max;
+  r.step_min = t.step_min;
+  r.ctol = t.corrector_tol;
+  r.contraction = t.contraction;
+  r.maxit = t.corrector_maxit;
+  r.grow_after = t.grow_after;
+  r.max_steps = t.max_steps;
+  r.use_endgame = o->use_endgame != 0;
+  r.t_start = e.t_start;
+  r.rf = e.radius_factor;
+  r.rmin = e.radius_min;
+  r.loop_radius = e.loop_radius;
+  r.agreement = e.winding_agreement;
+  r.egtol = e.tol;
+  r.rtol = e.residual_tol;
+  r.closure = e.closure_factor;
+  r.nsamples = e.nsamples;
+  r.max_winding = e.max_winding;
+  if (r.use_endgame && (e.nsamples < 1 || e.nsamples > 4096))
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "nsamples outside 1..4096");
+  // dtheta = 2 pi / nsamples and log(1/radius_factor) exactly as the
+  // reference computes them (src/endgame.f90:350, :225)
+  r.dtheta = 2.0 * 3.14159265358979323846 / (double)(e.nsamples > 0 ? e.nsamples : 1);
+  r.log_inv_rf = std::log(1.0 / e.radius_factor);
+  r.npaths = npaths;
+  h.npaths = npaths;
+  h.G = lanes_for(ctx, N);
+  return PHC_OK;
+}
+
+// (cos, sin) of theta0_j + (m / 2^TRIG_BITS) dtheta with the host libm,
+// theta0_j = j dtheta: the angles of loop arcs (src/endgame.f90:362-363,
+// src/tracker.f90:158-159) at dyadic segment parameters.
+int get_trig(phc_ctx* ctx, int nsamples, cx** out) {
+  auto it = ctx->trig.find(nsamples);
+  if (it != ctx->trig.end()) {
+    *out = it->second;
+    return PHC_OK;
+  }
+  const double dth = 2.0 * 3.14159265358979323846 / (double)nsamples;
+  std::vector<cx> tab((size_t)nsamples * (TRIG_M + 1));
+  for (int j = 0; j < nsamples; ++j) {
+    const double th0 = (double)j * dth;
+    for (int m = 0; m <= TRIG_M; ++m) {
+      const double s = (double)m / (double)TRIG_M;
+      const double th = th0 + s * dth;
+      tab[(size_t)j * (TRIG_M + 1) + m] = mk(std::cos(th), std::sin(th));
+    }
+  }
+  cx* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, tab.size() * sizeof(cx)));
+  HIPCHK(ctx, hipMemcpy(d, tab.data(), tab.size() * sizeof(cx),
+                        hipMemcpyHostToDevice));
+  ctx->trig[nsamples] = d;
+  *out = d;
+  return PHC_OK;
+}
+
+template <int G>
+hipError_t launch_track(const Launch& L, int blocks, int threads,
+                        size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL(track_kernel<G>, dim3(blocks), dim3(threads), lds, s, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_g(int G, const Launch& L, int blocks, int threads,
+                    size_t lds, hipStream_t s) {
+  switch (G) {
+#ifndef PHC_ONLY_G8
+    case 4:
+      return launch_track<4>(L, blocks, threads, lds, s);
+    case 16:
+      return launch_track<16>(L, blocks, threads, lds, s);
+#endif
+    default:
+      return launch_track<8>(L, blocks, threads, lds, s);
+  }
+}
+
+// A prepared multi-request run for one lanes class, with device buffers.
+struct ClassRun {
+  int G = 8;
+  std::vector<HostReq*> reqs;
+  Req* d_reqs = nullptr;
+  int64_t* d_base = nullptr;
+  unsigned long long* d_next = nullptr;
+  unsigned int* d_counters = nullptr;
+  unsigned int* d_active = nullptr;
+  PathState* d_slots = nullptr;
+  int64_t total = 0;
+  int expmax = 1;
+  int blocks = 0, threads = 0;
+  size_t lds = 0;
+};
+
+}  // namespace
+
+struct phc_staged {
+  Arena arena;
+  std::vector<std::unique_ptr<HostReq>> reqs;
+  std::vector<ClassRun> classes;
+  float last_ms = 0.f;
+};
+
+namespace {
+
+void* arena_take(Arena& a, size_t bytes) {
+  void* p = a.base + a.used;
+  a.used += Arena::al(bytes);
+  return p;
+}
+
+// Allocates device memory for all requests, uploads inputs, prepares one
+// ClassRun per lanes class.
+int stage_requests(phc_ctx* ctx, phc_staged* st) {
+  size_t need = 0;
+  const int N_any = 0;
+  (void)N_any;
+  std::map<int, std::vector<HostReq*>> byG;
+  for (auto& up : st->reqs) {
+    HostReq& h = *up;
+    const int N = h.r.N;
+    need += Arena::al(h.terms.size() * sizeof(TermRec));
+    need += Arena::al((size_t)h.npaths * N * sizeof(cx)) * 2;  // start, z
+    need += Arena::al((size_t)h.npaths * 4) * 8;
+    need += Arena::al((size_t)h.npaths * 8) * 3;
+    need += 256;
+    byG[h.G].push_back(&h);
+  }
+  for (auto& kv : byG) {
+    const int G = kv.first;
+    int64_t total = 0;
+    for (auto* h : kv.second) total += h->npaths;
+    need += Arena::al(kv.second.size() * sizeof(Req));
+    need += Arena::al((kv.second.size() + 1) * sizeof(int64_t));
+    need += 256 * 3;
+    // slots for the grid (upper bound)
+    const int per_wg = WG / G;
+    const int64_t groups = std::min<int64_t>(
+        std::max<int64_t>(total, 1),
+        (int64_t)std::max(ctx->num_cus, 1) * 16 * per_wg);
+    need += Arena::al((size_t)(groups + per_wg) * G * sizeof(PathState));
+  }
+  need += 4096;
+  HIPCHK(ctx, hipMalloc(&st->arena.base, need));
+  st->arena.size = need;
+
+  for (auto& kv : byG) {
+    ClassRun cr;
+    cr.G = kv.first;
+    cr.reqs = kv.second;
+    std::vector<Req> host_reqs;
+    std::vector<int64_t> base(1, 0);
+    for (HostReq* h : kv.second) {
+      const int N = h->r.N;
+      TermRec* dt = (TermRec*)arena_take(st->arena, h->terms.size() * sizeof(TermRec));
+      HIPCHK(ctx, hipMemcpyAsync(dt, h->terms.data(),
+                                 h->terms.size() * sizeof(TermRec),
+                                 hipMemcpyHostToDevice, ctx->stream));
+      h->r.terms = dt;
+      cx* ds = (cx*)arena_take(st->arena, (size_t)h->npaths * N * sizeof(cx));
+      if (h->npaths)
+        HIPCHK(ctx, hipMemcpyAsync(ds, h->start,
+                                   (size_t)h->npaths * N * sizeof(cx),
+                                   hipMemcpyHostToDevice, ctx->stream));
+      h->r.start = ds;
+      h->r.o_z = (cx*)arena_take(st->arena, (size_t)h->npaths * N * sizeof(cx));
+      h->r.o_status = (int32_t*)arena_take(st->arena, (size_t)h->npaths * 4);
+      h->r.o_wind = (int32_t*)arena_take(st->arena, (size_t)h->npaths * 4);
+      h->r.o_egs = (int32_t*)arena_take(st->arena, (size_t)h->npaths * 4);
+      h->r.o_acc = (int32_t*)arena_take(st->arena, (size_t)h->npaths * 4);
+      h->r.o_rej = (int32_t*)arena_take(st->arena, (size_t)h->npaths * 4);
+      h->r.o_nlin = (int32_t*)arena_take(st->arena, (size_t)h->npaths * 4);
+      h->r.o_nbe = (int32_t*)arena_take(st->arena, (size_t)h->npaths * 4);
+      h->r.o_done = (uint32_t*)arena_take(st->arena, 4);
+      h->r.o_t = (double*)arena_take(st->arena, (size_t)h->npaths * 8);
+      h->r.o_egerr = (double*)arena_take(st->arena, (size_t)h->npaths * 8);
+      h->r.o_egres = (double*)arena_take(st->arena, (size_t)h->npaths * 8);
+      HIPCHK(ctx, hipMemsetAsync(h->r.o_done, 0, 4, ctx->stream));
+      if (h->r.use_endgame) {
+        int rc = get_trig(ctx, h->r.nsamples, (cx**)&h->r.trig);
+        if (rc) return rc;
+      }
+      host_reqs.push_back(h->r);
+      base.push_back(base.back() + h->npaths);
+      cr.expmax = std::max(cr.expmax, h->r.maxexp);
+    }
+    cr.total = base.back();
+    cr.d_reqs = (Req*)arena_take(st->arena, host_reqs.size() * sizeof(Req));
+    HIPCHK(ctx, hipMemcpyAsync(cr.d_reqs, host_reqs.data(),
+                               host_reqs.size() * sizeof(Req),
+                               hipMemcpyHostToDevice, ctx->stream));
+    cr.d_base = (int64_t*)arena_take(st->arena, base.size() * sizeof(int64_t));
+    HIPCHK(ctx, hipMemcpyAsync(cr.d_base, base.data(),
+                               base.size() * sizeof(int64_t),
+                               hipMemcpyHostToDevice, ctx->stream));
+    cr.d_next = (unsigned long long*)arena_take(st->arena, 8);
+    cr.d_counters = (unsigned int*)arena_take(st->arena, 16);
+    cr.d_active = (unsigned int*)arena_take(st->arena, 4);
+    // Launch shape: enough groups for every path up to a persistent cap;
+    // small runs use single-wave workgroups so they spread over all CUs.
+    const int G = cr.G;
+    int threads = ctx->wg_threads ? ctx->wg_threads : WG;
+    const int64_t groups_needed = std::max<int64_t>(cr.total, 1);
+    const int64_t cus = std::max(ctx->num_cus, 1);
+    if (!ctx->wg_threads) {
+      threads = WG;
+      while (threads > 64 && (groups_needed + threads / G - 1) / (threads / G) < 2 * cus)
+        threads /= 2;
+    }
+    const int per_wg = threads / G;
+    int64_t blocks = (groups_needed + per_wg - 1) / per_wg;
+    blocks = std::min<int64_t>(blocks, cus * 16 * (WG / threads));
+    cr.blocks = (int)std::max<int64_t>(blocks, 1);
+    cr.threads = threads;
+    cr.lds = (size_t)per_wg * (cr.expmax + 1) * G * sizeof(cx);
+    cr.d_slots = (PathState*)arena_take(
+        st->arena, (size_t)cr.blocks * per_wg * G * sizeof(PathState));
+    st->classes.push_back(cr);
+  }
+  return PHC_OK;
+}
+
+int run_classes(phc_ctx* ctx, phc_staged* st) {
+  for (ClassRun& cr : st->classes) {
+    HIPCHK(ctx, hipMemsetAsync(cr.d_next, 0, 8, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(cr.d_counters, 0, 16, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(cr.d_active, 0, 4, ctx->stream));
+    for (HostReq* h : cr.reqs)
+      HIPCHK(ctx, hipMemsetAsync(h->r.o_done, 0, 4, ctx->stream));
+    Launch L;
+    memset(&L, 0, sizeof(L));
+    L.reqs = cr.d_reqs;
+    L.req_base = cr.d_base;
+    L.nreq = (int)cr.reqs.size();
+    L.expmax = cr.expmax;
+    L.next = cr.d_next;
+    L.total = cr.total;
+    L.slots = nullptr;
+    L.resume = 0;
+    L.max_units = 0;
+    L.active = cr.d_active;
+    L.counters = cr.d_counters;
+    HIPCHK(ctx, launch_g(cr.G, L, cr.blocks, cr.threads, cr.lds, ctx->stream));
+  }
+  return PHC_OK;
+}
+
+template <typename T>
+int fetch_array(phc_ctx* ctx, T* host, const T* dev, size_t count) {
+  if (!host || count == 0) return PHC_OK;
+  HIPCHK(ctx, hipMemcpyAsync(host, dev, count * sizeof(T),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  return PHC_OK;
+}
+
+int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
+  for (auto& up : st->reqs) {
+    HostReq& h = *up;
+    phc_endpoints* o = h.out;
+    if (!o) continue;
+    const size_t np = (size_t)h.npaths;
+    int rc = 0;
+    rc |= fetch_array(ctx, o->status, h.r.o_status, np);
+    rc |= fetch_array(ctx, o->t, h.r.o_t, np);
+    rc |= fetch_array(ctx, o->z, (const double*)h.r.o_z, np * h.r.N * 2);
+    rc |= fetch_array(ctx, o->winding, h.r.o_wind, np);
+    rc |= fetch_array(ctx, o->eg_status, h.r.o_egs, np);
+    rc |= fetch_array(ctx, o->eg_error, h.r.o_egerr, np);
+    rc |= fetch_array(ctx, o->eg_residual, h.r.o_egres, np);
+    rc |= fetch_array(ctx, o->naccepted, h.r.o_acc, np);
+    rc |= fetch_array(ctx, o->nrejected, h.r.o_rej, np);
+    rc |= fetch_array(ctx, o->nlin, h.r.o_nlin, np);
+    rc |= fetch_array(ctx, o->nbe, h.r.o_nbe, np);
+    if (rc) return PHC_ERR_DEVICE;
+  }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return PHC_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+void phc_default_options(phc_solve_options* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->tracker.step_init = 0.05;
+  o->tracker.step_max = 0.1;
+  o->tracker.step_min = 1.0e-14;
+  o->tracker.corrector_tol = 1.0e-10;
+  o->tracker.contraction = 0.5;
+  o->tracker.corrector_maxit = 3;
+  o->tracker.grow_after = 5;
+  o->tracker.max_steps = 20000;
+  o->endgame.t_start = 0.1;
+  o->endgame.radius_factor = 0.25;
+  o->endgame.radius_min = 1.0e-13;
+  o->endgame.loop_radius = 1.0e-5;
+  o->endgame.winding_agreement = 0.2;
+  o->endgame.tol = 1.0e-11;
+  o->endgame.residual_tol = 1.0e-8;
+  o->endgame.closure_factor = 0.1;
+  o->endgame.nsamples = 8;
+  o->endgame.max_winding = 16;
+  o->infinity_tol = 1.0e-8;
+  o->singular_cond = 1.0e10;
+  o->real_tol = 1.0e-8;
+  o->dedup_tol = 1.0e-8;
+  o->singular_dedup_tol = 1.0e-6;
+  o->use_endgame = 1;
+  o->seed = 1;
+  o->refine_maxit = 5;
+}
+
+int phc_ctx_create(int device, phc_ctx** out) {
+  if (!out) return PHC_ERR_INVALID_INPUT;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return PHC_ERR_DEVICE;
+  if (device < 0 || device >= ndev) return PHC_ERR_DEVICE;
+  std::unique_ptr<phc_ctx> ctx(new phc_ctx);
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) return PHC_ERR_DEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+    return PHC_ERR_DEVICE;
+  ctx->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+      hipSuccess)
+    return PHC_ERR_DEVICE;
+  *out = ctx.release();
+  return PHC_OK;
+}
+
+int phc_ctx_destroy(phc_ctx* ctx) {
+  if (!ctx) return PHC_OK;
+  (void)hipSetDevice(ctx->device);
+  for (auto& kv : ctx->trig) (void)hipFree(kv.second);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PHC_OK;
+}
+
+int phc_ctx_lanes_per_path(const phc_ctx* ctx, int N) {
+  return lanes_for(ctx, N);
+}
+
+const char* phc_ctx_last_error(const phc_ctx* ctx) {
+  return ctx ? ctx->err : "no context";
+}
+
+int phc_ctx_sync(phc_ctx* ctx) {
+  if (!ctx) return PHC_ERR_INVALID_INPUT;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return PHC_OK;
+}
+
+int phc_stage_create(phc_ctx* ctx, const phc_system* sys,
+                     const phc_homotopy* hom, const phc_solve_options* opts,
+                     int64_t npaths, const double* start, phc_staged** out) {
+  if (!ctx || !out) return PHC_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<phc_staged> st(new phc_staged);
+  std::unique_ptr<HostReq> h(new HostReq);
+  int rc = validate_and_pack(ctx, sys, hom, opts, npaths, *h);
+  if (rc) return rc;
+  h->start = start;
+  st->reqs.push_back(std::move(h));
+  rc = stage_requests(ctx, st.get());
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *out = st.release();
+  return PHC_OK;
+}
+
+int phc_stage_run(phc_ctx* ctx, phc_staged* st) {
+  if (!ctx || !st) return PHC_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return run_classes(ctx, st);
+}
+
+int phc_stage_fetch(phc_ctx* ctx, phc_staged* st, phc_endpoints* out) {
+  if (!ctx || !st || st->reqs.empty()) return PHC_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  st->reqs[0]->out = out;
+  return fetch_outputs(ctx, st);
+}
+
+int phc_stage_destroy(phc_ctx* ctx, phc_staged* st) {
+  if (ctx) (void)hipSetDevice(ctx->device);
+  delete st;
+  return PHC_OK;
+}
+
+int phc_track_paths(phc_ctx* ctx, const phc_system* sys,
+                    const phc_homotopy* hom, const phc_solve_options* opts,
+                    int64_t npaths, const double* start, phc_endpoints* out) {
+  if (!ctx) return PHC_ERR_DEVICE;
+  if (npaths == 0) return PHC_OK;
+  if (!start || !out) return fail(ctx, PHC_ERR_INVALID_INPUT, "null output");
+  phc_staged* st = nullptr;
+  int rc = phc_stage_create(ctx, sys, hom, opts, npaths, start, &st);
+  if (rc) return rc;
+  rc = phc_stage_run(ctx, st);
+  if (!rc) rc = phc_stage_fetch(ctx, st, out);
+  phc_stage_destroy(ctx, st);
+  return rc;
+}
+
+int phc_track_batch(phc_ctx* ctx, int32_t nsys, const phc_system* sys,
+                    const phc_homotopy* hom, const phc_solve_options* opts,
+                    const int64_t* npaths, const double* const* start,
+                    phc_endpoints* out) {
+  if (!ctx) return PHC_ERR_DEVICE;
+  if (nsys < 0 || (nsys > 0 && (!sys || !hom || !opts || !npaths || !start || !out)))
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "null batch input");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<phc_staged> st(new phc_staged);
+  for (int s = 0; s < nsys; ++s) {
+    if (npaths[s] == 0) continue;
+    std::unique_ptr<HostReq> h(new HostReq);
+    int rc = validate_and_pack(ctx, &sys[s], &hom[s], &opts[s], npaths[s], *h);
+    if (rc) return rc;
+    h->start = start[s];
+    h->out = &out[s];
+    st->reqs.push_back(std::move(h));
+  }
+  if (st->reqs.empty()) return PHC_OK;
+  int rc = stage_requests(ctx, st.get());
+  if (rc) return rc;
+  rc = run_classes(ctx, st.get());
+  if (rc) return rc;
+  return fetch_outputs(ctx, st.get());
+}
+
+}  // extern "C"

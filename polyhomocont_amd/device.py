@@ -1,0 +1,142 @@
+"""Python handle on libphc_hip.so (the C ABI of include/phc.h).
+
+Loads the in-tree library built by polyhomocont_amd/build.py and fails
+loudly when it is missing: there is no CPU fallback for sparse systems.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libphc_hip.so")
+
+_lib = None
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded C library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DeviceError(f"{LIB_PATH} is missing: build the HIP library "
+                              "first (python -c 'import __graft_entry__ as g; "
+                              "g.build()')")
+        L = C.CDLL(LIB_PATH)
+        P, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.phc_default_options.argtypes = [P]
+        L.phc_default_options.restype = None
+        L.phc_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
+        L.phc_ctx_destroy.argtypes = [P]
+        L.phc_ctx_lanes_per_path.argtypes = [P, C.c_int]
+        L.phc_ctx_last_error.argtypes = [P]
+        L.phc_ctx_last_error.restype = C.c_char_p
+        L.phc_ctx_sync.argtypes = [P]
+        L.phc_track_paths.argtypes = [P, P, P, P, i64, P, P]
+        L.phc_stage_create.argtypes = [P, P, P, P, i64, P, C.POINTER(P)]
+        L.phc_stage_run.argtypes = [P, P]
+        L.phc_stage_fetch.argtypes = [P, P, P]
+        L.phc_stage_destroy.argtypes = [P, P]
+        L.phc_track_batch.argtypes = [P, i32, P, P, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+class Context:
+    """A phc_ctx on one HIP device."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.phc_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise DeviceError(f"phc_ctx_create({device}) failed with {rc}: "
+                              "no usable GPU")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.phc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.L.phc_ctx_last_error(self.h).decode(errors="replace")
+            raise DeviceError(f"{what} failed with {rc}: {msg}")
+
+    def lanes_per_path(self, N):
+        return self.L.phc_ctx_lanes_per_path(self.h, N)
+
+    def track_paths(self, packed, hom, opts, start):
+        """Endpoint records for start points `start` [npaths][N]."""
+        start = np.ascontiguousarray(start, np.complex128)
+        npaths = start.shape[0]
+        out = abi.EndpointArrays(npaths, packed.n + 1)
+        rc = self.L.phc_track_paths(self.h, C.byref(packed.c), C.byref(hom.c),
+                                    C.byref(opts), npaths, start.ctypes.data,
+                                    C.byref(out.c))
+        self._check(rc, "phc_track_paths")
+        return out
+
+    def track_batch(self, items):
+        """items: list of (packed, hom, opts, start); returns EndpointArrays."""
+        n = len(items)
+        systems = (abi.System * n)(*[it[0].c for it in items])
+        homs = (abi.Homotopy * n)(*[it[1].c for it in items])
+        opts = (abi.SolveOptions * n)(*[it[2] for it in items])
+        starts = [np.ascontiguousarray(it[3], np.complex128) for it in items]
+        npaths = np.array([s.shape[0] for s in starts], np.int64)
+        sp = (C.c_void_p * n)(*[s.ctypes.data for s in starts])
+        outs = [abi.EndpointArrays(s.shape[0], it[0].n + 1)
+                for s, it in zip(starts, items)]
+        oc = (abi.Endpoints * n)(*[o.c for o in outs])
+        rc = self.L.phc_track_batch(self.h, n, systems, homs, opts,
+                                    npaths.ctypes.data, sp, oc)
+        self._check(rc, "phc_track_batch")
+        return outs
+
+
+class Staged:
+    """Inputs staged in device memory for repeated timed runs."""
+
+    def __init__(self, ctx, packed, hom, opts, start):
+        self.ctx = ctx
+        self.start = np.ascontiguousarray(start, np.complex128)
+        self.packed, self.hom, self.opts = packed, hom, opts
+        h = C.c_void_p()
+        rc = ctx.L.phc_stage_create(ctx.h, C.byref(packed.c), C.byref(hom.c),
+                                    C.byref(opts), self.start.shape[0],
+                                    self.start.ctypes.data, C.byref(h))
+        ctx._check(rc, "phc_stage_create")
+        self.h = h
+
+    def run(self):
+        self.ctx._check(self.ctx.L.phc_stage_run(self.ctx.h, self.h),
+                        "phc_stage_run")
+
+    def sync(self):
+        self.ctx._check(self.ctx.L.phc_ctx_sync(self.ctx.h), "phc_ctx_sync")
+
+    def fetch(self):
+        out = abi.EndpointArrays(self.start.shape[0], self.packed.n + 1)
+        self.ctx._check(self.ctx.L.phc_stage_fetch(self.ctx.h, self.h,
+                                                   C.byref(out.c)),
+                        "phc_stage_fetch")
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.L.phc_stage_destroy(self.ctx.h, self.h)
+            self.h = None
