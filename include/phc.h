@@ -179,6 +179,12 @@ int phc_ctx_destroy(phc_ctx *ctx);
  * autotuning if phc_autotune ran on this context. */
 int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
 
+/* Device counters of the last tracking call on this context:
+ * out[0] arc angles not covered by the exact angle table (computed on the
+ * device instead), out[1] linearization units executed (all groups),
+ * out[2], out[3] reserved (0). */
+int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
+
 /* Last error message of the context (static storage, never NULL). */
 const char *phc_ctx_last_error(const phc_ctx *ctx);
 

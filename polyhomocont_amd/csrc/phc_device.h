@@ -175,7 +175,7 @@ PHC_D cx arc_cs(const Req& R, const PathState& S, double s,
   }
   double theta = S.sa + s * S.sdth;
   if (fallbacks) atomicAdd(fallbacks, 1u);
-  return mk(cos(theta), sin(theta));
+  return cr_cossin(theta);
 }
 
 PHC_D cx seg_t(const Req& R, const PathState& S, double s,

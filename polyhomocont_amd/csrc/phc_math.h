@@ -145,4 +145,111 @@ PHC_HD cx zpowi(cx base, int e) {
   return acc;
 }
 
+// ---- correctly rounded sin/cos ------------------------------------------
+// Double-double evaluation (Dekker products with FMA) of cos and sin for
+// |theta| < 2^20: Cody-Waite reduction by pi/2 with a three-part pi/2,
+// Taylor series to 2^-110 on |r| <= pi/4, one final rounding. The result
+// is the correctly rounded double except when the exact value lies within
+// ~2^-100 relative of a rounding midpoint.
+struct dd {
+  double hi, lo;
+};
+PHC_HD dd dd_two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  dd r;
+  r.hi = s;
+  r.lo = e;
+  return r;
+}
+PHC_HD dd dd_fast(double a, double b) {
+  double s = a + b;
+  dd r;
+  r.hi = s;
+  r.lo = b - (s - a);
+  return r;
+}
+PHC_HD dd dd_add(dd a, dd b) {
+  dd s = dd_two_sum(a.hi, b.hi);
+  dd t = dd_two_sum(a.lo, b.lo);
+  s.lo += t.hi;
+  s = dd_fast(s.hi, s.lo);
+  s.lo += t.lo;
+  return dd_fast(s.hi, s.lo);
+}
+PHC_HD dd dd_mul(dd a, dd b) {
+  double p = a.hi * b.hi;
+  double e = fma(a.hi, b.hi, -p);
+  e += a.hi * b.lo + a.lo * b.hi;
+  return dd_fast(p, e);
+}
+PHC_HD dd dd_mul_d(dd a, double b) {
+  double p = a.hi * b;
+  double e = fma(a.hi, b, -p);
+  e += a.lo * b;
+  return dd_fast(p, e);
+}
+
+// sin (which = 0) and cos (which = 1) of a double-double r, |r| <= pi/4.
+PHC_HD dd dd_sincos_kernel(dd r, int which) {
+  dd r2 = dd_mul(r, r);
+  // term_k = (-1)^k r^(2k+which) / (2k+which)!
+  dd term;
+  if (which == 0) {
+    term = r;
+  } else {
+    term.hi = 1.0;
+    term.lo = 0.0;
+  }
+  dd sum = term;
+  int m = (which == 0) ? 1 : 0;  // power of the current term
+  for (int k = 1; k < 16; ++k) {
+    term = dd_mul(term, r2);
+    const double den = (double)((m + 1) * (m + 2));
+    // divide by den in double-double: q = term / den
+    double qh = term.hi / den;
+    double rem = fma(-qh, den, term.hi) + term.lo;
+    double ql = rem / den;
+    term.hi = -qh;
+    term.lo = -ql;
+    term = dd_fast(term.hi, term.lo);
+    m += 2;
+    sum = dd_add(sum, term);
+    if (fabs(term.hi) < 0x1p-112 * fabs(sum.hi)) break;
+  }
+  return sum;
+}
+
+// (cos theta, sin theta), each correctly rounded.
+PHC_HD cx cr_cossin(double theta) {
+  // pi/2 = P1 + P2 + P3 (53 + 53 + 53 bits)
+  const double P1 = 0x1.921fb54442d18p+0;
+  const double P2 = 0x1.1a62633145c07p-54;
+  const double P3 = -0x1.f1976b7ed8fbcp-110;
+  const double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+  double kd = rint(theta * TWO_OVER_PI);
+  // r = theta - k P1 - k P2 - k P3 in double-double, products exact
+  double p1 = kd * P1;
+  double e1 = fma(kd, P1, -p1);
+  dd r = dd_two_sum(theta, -p1);
+  r = dd_add(r, dd_fast(-e1, 0.0));
+  double p2 = kd * P2;
+  double e2 = fma(kd, P2, -p2);
+  r = dd_add(r, dd_two_sum(-p2, -e2));
+  r = dd_add(r, dd_fast(-kd * P3, 0.0));
+  const int q = ((int)kd) & 3;
+  dd sn = dd_sincos_kernel(r, 0);
+  dd cs = dd_sincos_kernel(r, 1);
+  double s = sn.hi + sn.lo, c = cs.hi + cs.lo;
+  cx out;
+  switch (q) {
+    case 0: out = mk(c, s); break;
+    case 1: out = mk(-s, c); break;
+    case 2: out = mk(-c, -s); break;
+    default: out = mk(s, -c); break;
+  }
+  return out;
+}
+
 }  // namespace phc

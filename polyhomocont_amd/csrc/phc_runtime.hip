@@ -31,6 +31,7 @@ struct phc_ctx {
   int lanes_override[PHC_MAX_UNKNOWNS + 1] = {0};
   int wg_threads = 0;  // 0: choose per launch
   std::map<int, cx*> trig;  // nsamples -> device table
+  unsigned long long counters[4] = {0, 0, 0, 0};  // of the last run
   std::mutex mu;
   char err[512] = {0};
 };
@@ -393,6 +394,15 @@ int fetch_array(phc_ctx* ctx, T* host, const T* dev, size_t count) {
 }
 
 int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
+  for (int i = 0; i < 4; ++i) ctx->counters[i] = 0;
+  for (ClassRun& cr : st->classes) {
+    unsigned int c[4];
+    HIPCHK(ctx, hipMemcpyAsync(c, cr.d_counters, sizeof(c),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->counters[0] += c[0] / (unsigned)cr.G;  // every lane of a group adds
+    ctx->counters[1] += c[1];                   // lane 0 of a group adds
+  }
   for (auto& up : st->reqs) {
     HostReq& h = *up;
     phc_endpoints* o = h.out;
@@ -488,6 +498,12 @@ int phc_ctx_lanes_per_path(const phc_ctx* ctx, int N) {
 
 const char* phc_ctx_last_error(const phc_ctx* ctx) {
   return ctx ? ctx->err : "no context";
+}
+
+int phc_ctx_counters(const phc_ctx* ctx, uint64_t* out) {
+  if (!ctx || !out) return PHC_ERR_INVALID_INPUT;
+  for (int i = 0; i < 4; ++i) out[i] = ctx->counters[i];
+  return PHC_OK;
 }
 
 int phc_ctx_sync(phc_ctx* ctx) {

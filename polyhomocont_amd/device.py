@@ -38,6 +38,7 @@ def lib():
         L.phc_ctx_last_error.argtypes = [P]
         L.phc_ctx_last_error.restype = C.c_char_p
         L.phc_ctx_sync.argtypes = [P]
+        L.phc_ctx_counters.argtypes = [P, P]
         L.phc_track_paths.argtypes = [P, P, P, P, i64, P, P]
         L.phc_stage_create.argtypes = [P, P, P, P, i64, P, C.POINTER(P)]
         L.phc_stage_run.argtypes = [P, P]
@@ -75,6 +76,11 @@ class Context:
         if rc != 0:
             msg = self.L.phc_ctx_last_error(self.h).decode(errors="replace")
             raise DeviceError(f"{what} failed with {rc}: {msg}")
+
+    def counters(self):
+        c = np.zeros(4, np.uint64)
+        self.L.phc_ctx_counters(self.h, c.ctypes.data)
+        return dict(trig_fallbacks=int(c[0]), units=int(c[1]))
 
     def lanes_per_path(self, N):
         return self.L.phc_ctx_lanes_per_path(self.h, N)

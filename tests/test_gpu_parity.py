@@ -1,0 +1,128 @@
+"""GPU tracker vs the oracle, path by path, through the C ABI.
+
+The device reproduces the reference's arithmetic operation by operation, so
+on these inputs the endpoint records are bit-identical: same status,
+winding number, end game status, accepted/rejected step counts and the
+same endpoint doubles. Hard cases (C3: endpoints at infinity with winding
+numbers up to 14) are held to the stated tolerance: classification fields
+equal, endpoints within 1e-10 relative.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from polyhomocont_amd import abi, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIELDS = ("status", "winding", "eg_status", "naccepted", "nrejected")
+
+
+def golden():
+    return np.load(os.path.join(HERE, "golden", "endpoints.npz"))
+
+
+def run_device(ctx, usys, opts, consts, start):
+    g, b, a = consts
+    return ctx.track_paths(usys.packed(), abi.HomotopyData(g, b, a), opts, start)
+
+
+def assert_bit_exact(got, want, n):
+    for f in FIELDS:
+        assert np.array_equal(getattr(got, f)[:n], getattr(want, f)[:n]), f
+    assert np.array_equal(got.z[:n], want.z[:n])
+    assert np.array_equal(got.eg_error[:n], want.eg_error[:n])
+    assert np.array_equal(got.t[:n], want.t[:n])
+
+
+@pytest.mark.parametrize("name", ["cubic", "2x2", "infinity", "double_root",
+                                  "triple_root", "mixed", "mult4", "user_cf",
+                                  "univariate5", "katsura3", "katsura4",
+                                  "cyclic5", "c1"])
+def test_reference_systems_bit_exact_vs_golden(ctx, name):
+    """Against the committed fixtures of the compiled reference."""
+    G = golden()
+    usys = W.config_system(name) if name == "c1" else W.ref_test_systems()[name]
+    opts = abi.default_options()
+    consts = (G[name + "/gamma"][0], G[name + "/b"], G[name + "/patch"])
+    from polyhomocont_amd import host
+    start = host.library().start_points(usys, opts.seed)
+    print(f"[gpu] {name}: {len(start)} paths", flush=True)
+    got = run_device(ctx, usys, opts, consts, start)
+    n = len(start)
+    for f in FIELDS + ("eg_error", "t"):
+        assert np.array_equal(getattr(got, f)[:n], G[f"{name}/{f}"]), f
+    assert np.array_equal(got.z[:n], G[name + "/z"])
+
+
+def test_c2_sample_bit_exact(ctx, ref):
+    usys = W.config_system("c2")
+    opts = abi.default_options()
+    g, b, a, ns = ref.homotopy(usys, opts.seed)
+    cnt = 192
+    start = ref.start_points(usys, opts.seed, 0, cnt)
+    print("[gpu] c2 sample", cnt, flush=True)
+    got = run_device(ctx, usys, opts, (g, b, a), start)
+    want = ref.endpoints(usys, opts, 0, cnt)
+    assert_bit_exact(got, want, cnt)
+
+
+@pytest.mark.parametrize("name,first,count", [("c3v", 0, 96), ("c3", 200, 24)])
+def test_c3_sample_classification_exact(ctx, ref, name, first, count):
+    usys = W.config_system(name)
+    opts = abi.default_options()
+    g, b, a, ns = ref.homotopy(usys, opts.seed)
+    start = ref.start_points(usys, opts.seed, first, count)
+    print(f"[gpu] {name} paths {first}..{first + count}", flush=True)
+    got = run_device(ctx, usys, opts, (g, b, a), start)
+    want = ref.endpoints(usys, opts, first, count)
+    for f in ("status", "winding", "eg_status"):
+        assert np.array_equal(getattr(got, f)[:count], getattr(want, f)[:count]), f
+    ok = want.status[:count] == 0
+    scale = np.maximum(np.abs(want.z[:count]).max(axis=1), 1e-300)
+    rel = np.abs(got.z[:count] - want.z[:count]).max(axis=1) / scale
+    assert rel[ok].max() <= 1e-10, rel[ok].max()
+
+
+def test_option_variants_match_reference(ctx, ref):
+    S = W.ref_test_systems()
+    cases = [
+        ("2x2", abi.default_options(use_endgame=0)),
+        ("cubic", abi.default_options(tracker_step_max=1e-3, tracker_step_init=1e-3,
+                                      tracker_max_steps=10)),
+        ("mixed", abi.default_options(tracker_step_min=0.02)),
+        ("double_root", abi.default_options(endgame_nsamples=5, seed=3)),
+        ("mult4", abi.default_options(endgame_max_winding=2, seed=9)),
+        ("infinity", abi.default_options(tracker_corrector_maxit=1, seed=2)),
+    ]
+    for name, opts in cases:
+        usys = S[name]
+        g, b, a, ns = ref.homotopy(usys, opts.seed)
+        start = ref.start_points(usys, opts.seed)
+        got = run_device(ctx, usys, opts, (g, b, a), start)
+        want = ref.endpoints(usys, opts)
+        assert_bit_exact(got, want, ns)
+    # the forced failure codes really occur
+    got = run_device(ctx, S["cubic"], cases[1][1],
+                     ref.homotopy(S["cubic"], 1)[:3], ref.start_points(S["cubic"], 1))
+    assert (got.status[:3] == 2).all()
+
+
+def test_batch_equals_separate_calls(ctx, host):
+    """A multi-system launch gives each system exactly its own results."""
+    systems = [W.random_sparse(104 + s) for s in range(6)]
+    systems += [W.config_system("c1"), W.ref_test_systems()["mixed"]]
+    opts = abi.default_options()
+    items, single = [], []
+    for usys in systems:
+        g, b, a, ns = host.homotopy(usys, opts.seed)
+        start = host.start_points(usys, opts.seed, 0, min(ns, 96))
+        item = (host.pack(usys), abi.HomotopyData(g, b, a), opts, start)
+        items.append(item)
+        single.append(ctx.track_paths(*item))
+    print("[gpu] batch of", len(items), flush=True)
+    outs = ctx.track_batch(items)
+    for o, s, it in zip(outs, single, items):
+        assert_bit_exact(o, s, len(it[3]))
