@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X path tracker.
+
+Workload (one step): one solve of C2 -- a single random dense system,
+n = 7, every equation of total degree 3 with all 120 monomials of degree
+<= 3, complex-Gaussian coefficients (seed 102), complex128 -- i.e. all
+2187 total-degree paths tracked from t = 1 through the Cauchy end game by
+the plain per-solve GPU tracker. Inputs (packed system, constants, start
+points) are resident in HBM before the timed region; the timed region is K
+back-to-back tracking runs.
+
+Reported beside the throughput:
+  roofline      FP64 work of the tracking kernel: canonical flops per
+                launch (polyhomocont_amd/flops.py, per-path linearization
+                counts from the device) over the kernel's average duration
+                measured with HIP events on the library's stream;
+  cpu_baseline  the compiled reference (oracle/_ref, flang -O2) on one host
+                core over a bounded sample of the same paths;
+  parity        endpoints of that sample vs the reference (max relative
+                difference; classification fields compared exactly).
+
+--gpus N > 1 is not supported: the tracker runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix), dense
+PEAK_HBM_GBS = 8000.0
+
+
+def load_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(usys, opts, npaths, budget_s):
+    import numpy as np
+    import oracle
+    ref = oracle.ref()
+    kind = "reference"
+    if ref is None:
+        ref = oracle.port()
+        kind = "port"
+    # calibrate on a few paths, then time a sample that fills the budget
+    t0 = time.perf_counter()
+    ref.endpoints(usys, opts, 0, 4)
+    per = max((time.perf_counter() - t0) / 4, 1e-4)
+    count = int(min(npaths, max(8, budget_s / per)))
+    t0 = time.perf_counter()
+    rec = ref.endpoints(usys, opts, 0, count)
+    dt = time.perf_counter() - t0
+    return dict(value=count / dt, unit="paths/s", cores=1, kind=kind,
+                sample=f"paths 1..{count} of the same solve, serial "
+                       f"(1 host core, {dt:.1f} s)"), rec, count
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported; the path "
+              "tracker runs on one GPU (paths would shard by index range "
+              "across GPUs with a host gather, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from polyhomocont_amd import abi, device, flops, host, workloads
+
+    usys = workloads.config_system(args.workload)
+    opts = abi.default_options()
+    H = host.library()
+    gamma, b, patch, npaths = H.homotopy(usys, opts.seed)
+    start = H.start_points(usys, opts.seed)
+    packed = H.pack(usys)
+    hom = abi.HomotopyData(gamma, b, patch)
+
+    ctx = device.Context(0)
+    st = device.Staged(ctx, packed, hom, opts, start)
+    for _ in range(args.warmup):
+        st.run()
+        st.sync()
+
+    kernel_ms = []
+    torch.cuda.synchronize()
+    st.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st.run()
+        kernel_ms.append(st.elapsed_ms())
+    st.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    rec = st.fetch()
+    info = st.launch_info()
+
+    value = npaths * args.steps / wall
+    flops_launch = int(flops.path_flops(packed, rec.nlin[:npaths],
+                                        rec.nbe[:npaths]).sum())
+    avg_s = float(np.mean(kernel_ms)) / 1e3
+    achieved = flops_launch / avg_s / 1e12
+    traffic = load_traffic(args.workload)
+
+    out = {
+        "metric": "paths/sec at endpoint err <=1e-10 vs CPU ref; p50/p99 "
+                  "solve latency @ fixed QPS",
+        "value": round(value, 3),
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (fp64)",
+        "data": "synthetic (seeded complex-Gaussian coefficients)",
+        "config": {"workload": f"{args.workload}: dense n=7, degree 3, 120 "
+                               "monomials/equation, 2187 paths, plain per-solve "
+                               "GPU tracker" if args.workload == "c2" else args.workload,
+                   "npaths": npaths, "lanes_per_path": info["lanes_per_path"],
+                   "workgroups": info["workgroups"],
+                   "threads_per_workgroup": info["threads"]},
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(achieved, 4),
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
+            "traffic": traffic,
+            "kernel": "phc::track_kernel<8>",
+            "flops_per_launch": flops_launch,
+            "avg_launch_ms": round(avg_s * 1e3, 4),
+            "note": "FP64 vector ALU work (no MFMA); peak is the MI355X "
+                    "FP64 figure",
+        },
+        "work": {"linearizations": int(rec.nlin[:npaths].sum()),
+                 "backward_errors": int(rec.nbe[:npaths].sum()),
+                 "steps_accepted": int(rec.naccepted[:npaths].sum()),
+                 "steps_rejected": int(rec.nrejected[:npaths].sum())},
+    }
+
+    if not args.no_cpu:
+        cb, want, cnt = cpu_baseline(usys, opts, npaths, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        same = all(np.array_equal(getattr(rec, f)[:cnt], getattr(want, f)[:cnt])
+                   for f in ("status", "winding", "eg_status"))
+        scale = np.maximum(np.abs(want.z[:cnt]).max(axis=1), 1e-300)
+        err = float((np.abs(rec.z[:cnt] - want.z[:cnt]).max(axis=1) / scale).max())
+        bit = float((rec.z[:cnt] == want.z[:cnt]).all(axis=1).mean())
+        out["parity"] = {"paths_compared": cnt, "max_rel_endpoint_err": err,
+                         "classification_equal": bool(same),
+                         "bit_identical_fraction": bit}
+        out["gpu_vs_cpu_core"] = round(value / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

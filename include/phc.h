@@ -207,6 +207,13 @@ int phc_stage_create(phc_ctx *ctx, const phc_system *sys,
 int phc_stage_run(phc_ctx *ctx, phc_staged *st);
 int phc_stage_fetch(phc_ctx *ctx, phc_staged *st, phc_endpoints *out);
 int phc_stage_destroy(phc_ctx *ctx, phc_staged *st);
+/* Device time (ms) of the tracking kernels of the last phc_stage_run,
+ * measured with HIP events recorded on the context stream around each
+ * launch; waits for the run to finish. */
+int phc_stage_elapsed(phc_ctx *ctx, phc_staged *st, double *ms);
+/* Launch shape of the first lanes class: info[0] lanes per path, [1]
+ * workgroups, [2] threads per workgroup, [3] LDS bytes per workgroup. */
+int phc_stage_launch_info(phc_ctx *ctx, phc_staged *st, int32_t *info);
 int phc_ctx_sync(phc_ctx *ctx);
 
 /* ---- batched multi-system tracker ------------------------------------- */

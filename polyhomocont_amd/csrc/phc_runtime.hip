@@ -230,6 +230,7 @@ struct ClassRun {
   int expmax = 1;
   int blocks = 0, threads = 0;
   size_t lds = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 }  // namespace
@@ -238,7 +239,12 @@ struct phc_staged {
   Arena arena;
   std::vector<std::unique_ptr<HostReq>> reqs;
   std::vector<ClassRun> classes;
-  float last_ms = 0.f;
+  ~phc_staged() {
+    for (ClassRun& cr : classes) {
+      if (cr.ev0) (void)hipEventDestroy(cr.ev0);
+      if (cr.ev1) (void)hipEventDestroy(cr.ev1);
+    }
+  }
 };
 
 namespace {
@@ -380,7 +386,11 @@ int run_classes(phc_ctx* ctx, phc_staged* st) {
     L.max_units = 0;
     L.active = cr.d_active;
     L.counters = cr.d_counters;
+    if (!cr.ev0) HIPCHK(ctx, hipEventCreate(&cr.ev0));
+    if (!cr.ev1) HIPCHK(ctx, hipEventCreate(&cr.ev1));
+    HIPCHK(ctx, hipEventRecord(cr.ev0, ctx->stream));
     HIPCHK(ctx, launch_g(cr.G, L, cr.blocks, cr.threads, cr.lds, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(cr.ev1, ctx->stream));
   }
   return PHC_OK;
 }
@@ -544,6 +554,32 @@ int phc_stage_fetch(phc_ctx* ctx, phc_staged* st, phc_endpoints* out) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   st->reqs[0]->out = out;
   return fetch_outputs(ctx, st);
+}
+
+int phc_stage_elapsed(phc_ctx* ctx, phc_staged* st, double* ms) {
+  if (!ctx || !st || !ms) return PHC_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  double total = 0.0;
+  for (ClassRun& cr : st->classes) {
+    if (!cr.ev0 || !cr.ev1) continue;
+    HIPCHK(ctx, hipEventSynchronize(cr.ev1));
+    float t = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&t, cr.ev0, cr.ev1));
+    total += t;
+  }
+  *ms = total;
+  return PHC_OK;
+}
+
+int phc_stage_launch_info(phc_ctx* ctx, phc_staged* st, int32_t* info) {
+  if (!ctx || !st || !info) return PHC_ERR_INVALID_INPUT;
+  if (st->classes.empty()) return PHC_ERR_INVALID_INPUT;
+  const ClassRun& cr = st->classes[0];
+  info[0] = cr.G;
+  info[1] = cr.blocks;
+  info[2] = cr.threads;
+  info[3] = (int32_t)cr.lds;
+  return PHC_OK;
 }
 
 int phc_stage_destroy(phc_ctx* ctx, phc_staged* st) {

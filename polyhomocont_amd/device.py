@@ -44,6 +44,8 @@ def lib():
         L.phc_stage_run.argtypes = [P, P]
         L.phc_stage_fetch.argtypes = [P, P, P]
         L.phc_stage_destroy.argtypes = [P, P]
+        L.phc_stage_elapsed.argtypes = [P, P, P]
+        L.phc_stage_launch_info.argtypes = [P, P, P]
         L.phc_track_batch.argtypes = [P, i32, P, P, P, P, P, P]
         _lib = L
     return _lib
@@ -141,6 +143,22 @@ class Staged:
                                                    C.byref(out.c)),
                         "phc_stage_fetch")
         return out
+
+    def elapsed_ms(self):
+        """Kernel time of the last run (HIP events on the library stream)."""
+        ms = np.zeros(1)
+        self.ctx._check(self.ctx.L.phc_stage_elapsed(self.ctx.h, self.h,
+                                                     ms.ctypes.data),
+                        "phc_stage_elapsed")
+        return float(ms[0])
+
+    def launch_info(self):
+        info = np.zeros(4, np.int32)
+        self.ctx._check(self.ctx.L.phc_stage_launch_info(self.ctx.h, self.h,
+                                                         info.ctypes.data),
+                        "phc_stage_launch_info")
+        return dict(lanes_per_path=int(info[0]), workgroups=int(info[1]),
+                    threads=int(info[2]), lds_bytes=int(info[3]))
 
     def close(self):
         if self.h:
