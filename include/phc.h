@@ -212,7 +212,8 @@ int phc_stage_destroy(phc_ctx *ctx, phc_staged *st);
  * launch; waits for the run to finish. */
 int phc_stage_elapsed(phc_ctx *ctx, phc_staged *st, double *ms);
 /* Launch shape of the first lanes class: info[0] lanes per path, [1]
- * workgroups, [2] threads per workgroup, [3] LDS bytes per workgroup. */
+ * workgroups, [2] threads per workgroup, [3] LDS bytes per workgroup,
+ * [4] 1 if the system program is staged in LDS. */
 int phc_stage_launch_info(phc_ctx *ctx, phc_staged *st, int32_t *info);
 int phc_ctx_sync(phc_ctx *ctx);
 

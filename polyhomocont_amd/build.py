@@ -96,6 +96,8 @@ def build_oracle():
     ref = os.environ.get("PHC_REFERENCE", "/root/reference")
     if os.path.isdir(os.path.join(ref, "src")):
         targets.append("ref")
+        if os.path.exists(os.path.join(LIB, "libpolyhomocont_amd.so")):
+            targets.append("droptests")
     if targets:
         _run(["make", "-s", "-C", odir, f"REF={ref}", *targets])
 

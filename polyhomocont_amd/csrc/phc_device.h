@@ -56,7 +56,7 @@ struct TermRec {
 
 // One solve request: the packed system, its random constants, options and
 // output arrays (all device pointers).
-struct Req {
+struct alignas(16) Req {
   int32_t n, N, maxexp, nterms;
   int32_t first[PHC_MAX_UNKNOWNS + 1];
   int32_t deg[PHC_MAX_UNKNOWNS];
@@ -132,6 +132,10 @@ struct Launch {
 };
 
 // ---------------------------------------------------------------- group ops
+// Broadcasts from a runtime lane of the group use ds_bpermute (__shfl);
+// reductions over the group use DPP lane swizzles inside a row of 16
+// lanes: row_ror:8 (16-lane groups), row_half_mirror, then quad_perm xor 1
+// and xor 2 leave every lane of the group with the reduced value.
 template <int G>
 PHC_D double gshfl(double v, int src) {
   return __shfl(v, src, G);
@@ -144,12 +148,52 @@ template <int G>
 PHC_D cx gshfl_c(cx v, int src) {
   return mk(__shfl(v.re, src, G), __shfl(v.im, src, G));
 }
+
+template <int CTRL>
+PHC_D int dpp_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+PHC_D double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i<CTRL>((int)(b & 0xffffffffll));
+  const int hi = dpp_i<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2,3,0,1]
+constexpr int DPP_HALF_MIRROR = 0x141; // lane i <-> 7 - i within 8 lanes
+constexpr int DPP_ROR8 = 0x128;        // lane i <- (i + 8) mod 16
+
 template <int G>
 PHC_D double gmax(double v) {
-#pragma unroll
-  for (int m = 1; m < G; m <<= 1) v = fmax(v, __shfl_xor(v, m, G));
+  if (G >= 16) v = fmax(v, dpp_d<DPP_ROR8>(v));
+  if (G >= 8) v = fmax(v, dpp_d<DPP_HALF_MIRROR>(v));
+  v = fmax(v, dpp_d<DPP_XOR1>(v));
+  v = fmax(v, dpp_d<DPP_XOR2>(v));
   return v;
 }
+
+// Arg-max over the group: largest v, ties to the smallest p (then any l).
+template <int CTRL>
+PHC_D void argmax_step(double& v, int& p, int& l) {
+  const double ov = dpp_d<CTRL>(v);
+  const int op = dpp_i<CTRL>(p);
+  const int ol = dpp_i<CTRL>(l);
+  if (ov > v || (ov == v && op < p)) {
+    v = ov;
+    p = op;
+    l = ol;
+  }
+}
+template <int G>
+PHC_D void gargmax(double& v, int& p, int& l) {
+  if (G >= 16) argmax_step<DPP_ROR8>(v, p, l);
+  if (G >= 8) argmax_step<DPP_HALF_MIRROR>(v, p, l);
+  argmax_step<DPP_XOR1>(v, p, l);
+  argmax_step<DPP_XOR2>(v, p, l);
+}
+
 // max_i |v_i| over the N components held by the group (src/linalg.f90:176).
 template <int G>
 PHC_D double gnorm(cx v, int lane, int N) {
@@ -196,15 +240,39 @@ PHC_D cx seg_dt(const Req& R, const PathState& S, double s,
 }
 
 // -------------------------------------------------------------------- unit
+// LDS of one group: the power table pw(e, j) at T[e * G + j] (e >= 1), the
+// N x N matrix at A[r * G + j], pivot lanes at PL[k] and the solution at
+// X[j]. All indices inside a group are runtime values, so the LU and the
+// triangular solves are compact loops rather than unrolled code.
+template <int G>
+struct GroupLds {
+  cx* T;
+  cx* A;
+  cx* X;
+  int* PL;
+  PHC_D static size_t bytes(int expmax) {
+    return ((size_t)(expmax + 1) * G + (size_t)G * G + G) * sizeof(cx) +
+           (size_t)G * sizeof(int);
+  }
+  PHC_D void bind(char* base, int expmax) {
+    T = (cx*)base;
+    A = T + (size_t)(expmax + 1) * G;
+    X = A + (size_t)G * G;
+    PL = (int*)(X + G);
+  }
+};
+
 // Evaluates H, H_z, H_t at (zq, t) and, for PC_TAN / PC_NEWT, solves
 // H_z x = -(H_t dt) / -H. Returns x_r in lane r. For PC_BE returns the
 // backward error of src/endgame.f90:269-285 in *be.
 template <int G>
-PHC_D void unit(const Req& R, const int lane, const int mode, const cx zq,
-                const cx t, const cx dt, cx* __restrict__ T, cx& xout,
-                int& okout, double& be) {
+PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
+                const int lane, const int mode, const cx zq, const cx t,
+                const cx dt, const GroupLds<G>& L, cx& xout, int& okout,
+                double& be) {
   const int N = R.N, n = R.n;
   const bool row = lane < N;
+  cx* __restrict__ T = L.T;
 
   // 1. power table pw(e, j) = zq_j^e, e = 1..maxexp (src/sparse.f90:252-255)
   wave_lds_fence();
@@ -228,8 +296,11 @@ PHC_D void unit(const Req& R, const int lane, const int mode, const cx zq,
     k0 = R.first[lane];
     k1 = R.first[lane + 1];
   }
+  TermRec nxt;
+  if (k0 < k1) nxt = terms[k0];
   for (int k = k0; k < k1; ++k) {
-    const TermRec tr = R.terms[k];
+    const TermRec tr = nxt;
+    if (k + 1 < k1) nxt = terms[k + 1];  // prefetch the next term
     const cx c = mk(tr.re, tr.im);
     int ex[G];
 #pragma unroll
@@ -325,51 +396,40 @@ PHC_D void unit(const Req& R, const int lane, const int mode, const cx zq,
   // right-hand side: tangent -(H_t dt/ds), Newton -H (src/tracker.f90:314,348)
   const cx b = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
 
-  // 4. LU with partial pivoting, logical row swaps (src/linalg.f90:49-76)
+  // 4. LU with partial pivoting in LDS, logical row swaps
+  //    (src/linalg.f90:49-76): lane r keeps physical row r; pos is its
+  //    logical position, pivot ties go to the smallest logical position.
+  cx* __restrict__ A = L.A;
+  cx* const arow = A + lane * G;
+  if (row) {
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (j < N) arow[j] = hz[j];
+  }
   int pos = row ? lane : 1 << 20;
-  int pl[G];
   int ok = 1;
-#pragma unroll
-  for (int k = 0; k < G; ++k) {
-    pl[k] = 0;
-    if (k < N && ok) {
-      double bv = (row && pos >= k) ? cabs_(hz[k]) : -1.0;
-      int bp = pos, bl = lane;
-#pragma unroll
-      for (int m = 1; m < G; m <<= 1) {
-        const double ov = __shfl_xor(bv, m, G);
-        const int op = __shfl_xor(bp, m, G);
-        const int ol = __shfl_xor(bl, m, G);
-        if (ov > bv || (ov == bv && op < bp)) {
-          bv = ov;
-          bp = op;
-          bl = ol;
-        }
-      }
-      if (bv == 0.0) {
-        ok = 0;
-      } else {
-        pl[k] = bl;
-        int np = pos;
-        if (pos == k) np = bp;
-        if (lane == bl) np = k;
-        pos = np;
-        const cx piv = gshfl_c<G>(hz[k], bl);
-        const bool below = row && pos > k;
-        cx l = hz[k];
-        if (below) {
-          l = cdiv(hz[k], piv);
-          hz[k] = l;
-        }
-#pragma unroll
-        for (int j = k + 1; j < G; ++j) {
-          if (j < N) {
-            const cx pj = gshfl_c<G>(hz[j], bl);
-            if (below) hz[j] = csub(hz[j], cmul(l, pj));
-          }
-        }
-      }
+  wave_lds_fence();
+  for (int k = 0; k < N; ++k) {
+    const cx ak = row ? arow[k] : mk(0.0, 0.0);
+    double bv = (row && pos >= k) ? cabs_(ak) : -1.0;
+    int bp = pos, bl = lane;
+    gargmax<G>(bv, bp, bl);
+    if (bv == 0.0) {
+      ok = 0;
+      break;
     }
+    if (lane == 0) L.PL[k] = bl;
+    int np = pos;
+    if (pos == k) np = bp;
+    if (lane == bl) np = k;
+    pos = np;
+    const cx* const prow = A + bl * G;
+    if (row && pos > k) {
+      const cx l = cdiv(ak, prow[k]);
+      arow[k] = l;
+      for (int j = k + 1; j < N; ++j) arow[j] = csub(arow[j], cmul(l, prow[j]));
+    }
+    wave_lds_fence();
   }
   okout = ok;
   if (!ok) {
@@ -380,42 +440,35 @@ PHC_D void unit(const Req& R, const int lane, const int mode, const cx zq,
   // 5. forward substitution with unit L in logical row order
   cx sacc = mk(0.0, 0.0);
   bool sset = false;
-#pragma unroll
-  for (int k = 0; k < G; ++k) {
-    if (k < N) {
-      const cx cand = (pos == 0 || !sset) ? b : csub(b, sacc);
-      const cx yk = gshfl_c<G>(cand, pl[k]);
-      if (row && pos > k) {
-        const cx p = cmul(hz[k], yk);
-        sacc = sset ? cadd(sacc, p) : p;
-        sset = true;
-      }
+  for (int k = 0; k < N; ++k) {
+    const cx cand = (pos == 0 || !sset) ? b : csub(b, sacc);
+    const cx yk = gshfl_c<G>(cand, L.PL[k]);
+    if (row && pos > k) {
+      const cx p = cmul(arow[k], yk);
+      sacc = sset ? cadd(sacc, p) : p;
+      sset = true;
     }
   }
   const cx yfin = (pos == 0 || !sset) ? b : csub(b, sacc);
-  // back substitution, sums in increasing column order
-  cx xs[G];
+  // back substitution, sums in increasing column order; the row at
+  // logical position i is owned by lane PL[i]
+  cx* __restrict__ X = L.X;
   cx xmine = mk(0.0, 0.0);
-#pragma unroll
-  for (int i = G - 1; i >= 0; --i) {
-    xs[i] = mk(0.0, 0.0);
-    if (i < N) {
-      cx s = mk(0.0, 0.0);
-      bool ss = false;
-#pragma unroll
-      for (int j = i + 1; j < G; ++j) {
-        if (j < N) {
-          const cx p = cmul(hz[j], xs[j]);
-          s = ss ? cadd(s, p) : p;
-          ss = true;
-        }
+  for (int i = N - 1; i >= 0; --i) {
+    cx cand = mk(0.0, 0.0);
+    if (row) {
+      cx num = yfin;
+      if (i + 1 < N) {
+        cx s = cmul(arow[i + 1], X[i + 1]);
+        for (int j = i + 2; j < N; ++j) s = cadd(s, cmul(arow[j], X[j]));
+        num = csub(yfin, s);
       }
-      const cx num = ss ? csub(yfin, s) : yfin;
-      const cx cand = cdiv(num, hz[i]);
-      const cx xi = gshfl_c<G>(cand, pl[i]);
-      xs[i] = xi;
-      if (lane == i) xmine = xi;
+      cand = cdiv(num, arow[i]);
     }
+    const cx xi = gshfl_c<G>(cand, L.PL[i]);
+    if (lane == 0) X[i] = xi;
+    if (lane == i) xmine = xi;
+    wave_lds_fence();
   }
   xout = xmine;
 }
@@ -866,14 +919,38 @@ struct Machine {
 };
 
 // --------------------------------------------------------------------- kernel
-template <int G>
+// STAGED: the launch has one request; its descriptor and its monomial
+// program are copied into LDS once per workgroup and every group reads
+// them from there. Otherwise requests and programs are read from global
+// memory (L2).
+constexpr size_t staged_req_bytes() { return (sizeof(Req) + 15) & ~size_t(15); }
+
+template <int G, bool STAGED>
 __global__ void __launch_bounds__(WG)
     track_kernel(Launch L) {
-  extern __shared__ cx lds[];
+  extern __shared__ __align__(16) char lds_raw[];
   const int lane = threadIdx.x % G;
   const int gi = threadIdx.x / G;
-  const int slot = blockIdx.x * (WG / G) + gi;
-  cx* T = lds + (size_t)gi * (L.expmax + 1) * G;
+  const int slot = blockIdx.x * (blockDim.x / G) + gi;
+
+  Req* sreq = (Req*)lds_raw;
+  TermRec* sterms = (TermRec*)(lds_raw + staged_req_bytes());
+  size_t group_base = 0;
+  if (STAGED) {
+    const uint4* src = (const uint4*)&L.reqs[0];
+    uint4* dst = (uint4*)sreq;
+    for (int i = threadIdx.x; i < (int)(sizeof(Req) / 16); i += blockDim.x)
+      dst[i] = src[i];
+    const int nt = L.reqs[0].nterms;
+    const uint4* tsrc = (const uint4*)L.reqs[0].terms;
+    uint4* tdst = (uint4*)sterms;
+    for (int i = threadIdx.x; i < nt * 2; i += blockDim.x) tdst[i] = tsrc[i];
+    __syncthreads();
+    group_base = staged_req_bytes() + (size_t)nt * sizeof(TermRec);
+  }
+  GroupLds<G> gl;
+  gl.bind(lds_raw + group_base + (size_t)gi * GroupLds<G>::bytes(L.expmax),
+          L.expmax);
 
   PathState S;
   if (L.resume) {
@@ -889,48 +966,51 @@ __global__ void __launch_bounds__(WG)
       if (lane == 0) idx = atomicAdd(L.next, 1ull);
       idx = __shfl((long long)idx, 0, G);
       if ((long long)idx >= L.total) break;
-      int q = 0;
-      int lo = 0, hi = L.nreq - 1;
-      while (lo < hi) {  // last request with base <= idx
-        const int mid = (lo + hi + 1) >> 1;
-        if (L.req_base[mid] <= (long long)idx)
-          lo = mid;
-        else
-          hi = mid - 1;
+      int lo = 0;
+      if (!STAGED) {
+        int hi = L.nreq - 1;
+        while (lo < hi) {  // last request with base <= idx
+          const int mid = (lo + hi + 1) >> 1;
+          if (L.req_base[mid] <= (long long)idx)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
       }
-      q = lo;
-      S.req = q;
-      S.path = (int32_t)((long long)idx - L.req_base[q]);
-      const Req& R = L.reqs[q];
+      S.req = lo;
+      S.path = (int32_t)((long long)idx - L.req_base[lo]);
+      const Req& R = STAGED ? *sreq : L.reqs[lo];
       Machine<G> M{R, S, lane, fb};
-      const cx z0 = (lane < R.N) ? R.start[(size_t)S.path * R.N + lane]
-                                 : mk(0.0, 0.0);
-      M.start_path(z0);
-      M.run();
-      if (S.pc == PC_IDLE) continue;
+      M.start_path((lane < R.N) ? R.start[(size_t)S.path * R.N + lane]
+                                : mk(0.0, 0.0));
+    } else {
+      if (L.max_units && units >= L.max_units) break;
+      const Req& R = STAGED ? *sreq : L.reqs[S.req];
+      Machine<G> M{R, S, lane, fb};
+      cx t = mk(0.0, 0.0), dt = mk(0.0, 0.0);
+      if (S.pc != PC_BE) {
+        const double sv = (S.pc == PC_TAN) ? S.se : S.sn;
+        t = seg_t(R, S, sv, fb);
+        if (S.pc == PC_TAN)
+          dt = (S.sk == 0) ? mk(S.sb - S.sa, 0.0)
+                           : mk(0.0 * t.re - S.sdth * t.im,
+                                0.0 * t.im + S.sdth * t.re);
+      }
+      cx x;
+      int ok;
+      double be = 0.0;
+      unit<G>(R, STAGED ? sterms : R.terms, lane, S.pc,
+              (lane < R.N) ? S.zq : mk(0.0, 0.0), t, dt, gl, x, ok, be);
+      units++;
+      if (S.pc == PC_TAN)
+        M.on_tangent(x, ok);
+      else if (S.pc == PC_NEWT)
+        M.on_newton(x, ok);
+      else
+        M.on_backward_error(be);
     }
-    if (L.max_units && units >= L.max_units) break;
-    const Req& R = L.reqs[S.req];
+    const Req& R = STAGED ? *sreq : L.reqs[S.req];
     Machine<G> M{R, S, lane, fb};
-    cx t = mk(0.0, 0.0), dt = mk(0.0, 0.0);
-    if (S.pc == PC_TAN) {
-      t = seg_t(R, S, S.se, fb);
-      dt = seg_dt(R, S, S.se, fb);
-    } else if (S.pc == PC_NEWT) {
-      t = seg_t(R, S, S.sn, fb);
-    }
-    cx x;
-    int ok;
-    double be = 0.0;
-    unit<G>(R, lane, S.pc, (lane < R.N) ? S.zq : mk(0.0, 0.0), t, dt, T, x,
-            ok, be);
-    units++;
-    if (S.pc == PC_TAN)
-      M.on_tangent(x, ok);
-    else if (S.pc == PC_NEWT)
-      M.on_newton(x, ok);
-    else
-      M.on_backward_error(be);
     M.run();
   }
   if (L.slots) L.slots[(size_t)slot * G + lane] = S;

@@ -27,8 +27,11 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define PHC_HD __host__ __device__ __forceinline__
+// rare slow paths stay out of line to keep the kernel's code small
+#define PHC_HD_COLD __host__ __device__ __noinline__
 #else
 #define PHC_HD static inline
+#define PHC_HD_COLD static
 #endif
 
 namespace phc {
@@ -52,20 +55,11 @@ PHC_HD cx cmul(cx a, cx b) {
 // (s, 0) * a for a real (or integer-valued) scalar s.
 PHC_HD cx cscale(double s, cx a) { return mk(s * a.re, s * a.im); }
 
-// compiler-rt __divdc3.
-PHC_HD cx cdiv(cx num, cx den) {
-  double a = num.re, b = num.im, c = den.re, d = den.im;
-  int ilogbw = 0;
-  double logbw = logb(fmax(fabs(c), fabs(d)));
-  if (isfinite(logbw)) {
-    ilogbw = (int)logbw;
-    c = ldexp(c, -ilogbw);
-    d = ldexp(d, -ilogbw);
-  }
-  double denom = c * c + d * d;
-  double x = ldexp((a * c + b * d) / denom, -ilogbw);
-  double y = ldexp((b * c - a * d) / denom, -ilogbw);
-  if (isnan(x) && isnan(y)) {
+// C99 Annex G recovery of infinities and zeros for __divdc3 when the
+// formula gave NaN + i NaN (c, d already scaled by 2^-ilogbw).
+PHC_HD_COLD cx cdiv_special(double a, double b, double c, double d,
+                            double denom, double logbw, double x, double y) {
+  {
     if (denom == 0.0 && (!isnan(a) || !isnan(b))) {
       x = copysign(INFINITY, c) * a;
       y = copysign(INFINITY, c) * b;
@@ -81,6 +75,25 @@ PHC_HD cx cdiv(cx num, cx den) {
       y = 0.0 * (b * c - a * d);
     }
   }
+  return mk(x, y);
+}
+
+// compiler-rt __divdc3.
+PHC_HD cx cdiv(cx num, cx den) {
+  const double a = num.re, b = num.im;
+  double c = den.re, d = den.im;
+  int ilogbw = 0;
+  const double logbw = logb(fmax(fabs(c), fabs(d)));
+  if (isfinite(logbw)) {
+    ilogbw = (int)logbw;
+    c = ldexp(c, -ilogbw);
+    d = ldexp(d, -ilogbw);
+  }
+  const double denom = c * c + d * d;
+  const double x = ldexp((a * c + b * d) / denom, -ilogbw);
+  const double y = ldexp((b * c - a * d) / denom, -ilogbw);
+  if (isnan(x) && isnan(y))
+    return cdiv_special(a, b, c, d, denom, logbw, x, y);
   return mk(x, y);
 }
 
@@ -101,28 +114,32 @@ PHC_HD double hypot_kernel(double ax, double ay) {
   return h;
 }
 
-// |a| with glibc hypot semantics, including the scaled ranges.
-PHC_HD double cabs_(cx a) {
-  double x = fabs(a.re), y = fabs(a.im);
+// hypot outside the common range: non-finite, huge or tiny operands.
+PHC_HD_COLD double cabs_special(double x, double y) {
   if (!isfinite(x) || !isfinite(y)) {
-    if ((isinf(x) || isinf(y)) ) return INFINITY;
+    if (isinf(x) || isinf(y)) return INFINITY;
     return x + y;  // NaN
   }
-  double ax = x < y ? y : x;
-  double ay = x < y ? x : y;
-  const double LARGE_VAL = 0x1p+511;
-  const double TINY_VAL = 0x1p-511;
+  const double ax = x < y ? y : x;
+  const double ay = x < y ? x : y;
   const double EPS = 0x1p-54;
   const double SCALE = 0x1p-600;
-  if (ax > LARGE_VAL) {
+  if (ax > 0x1p+511) {
     if (ay <= ax * EPS) return ax + ay;
     return hypot_kernel(ax * SCALE, ay * SCALE) / SCALE;
   }
-  if (ay < TINY_VAL) {
-    if (ax >= ay / EPS) return ax + ay;
-    return hypot_kernel(ax / SCALE, ay / SCALE) * SCALE;
-  }
-  if (ay <= ax * EPS) return ax + ay;
+  if (ax >= ay / EPS) return ax + ay;
+  return hypot_kernel(ax / SCALE, ay / SCALE) * SCALE;
+}
+
+// |a| with glibc hypot semantics.
+PHC_HD double cabs_(cx a) {
+  const double x = fabs(a.re), y = fabs(a.im);
+  const double ax = x < y ? y : x;
+  const double ay = x < y ? x : y;
+  // common range: finite, ax <= 2^511, ay >= 2^-511
+  if (!(ax <= 0x1p+511) || !(ay >= 0x1p-511)) return cabs_special(x, y);
+  if (ay <= ax * 0x1p-54) return ax + ay;
   return hypot_kernel(ax, ay);
 }
 
@@ -222,7 +239,7 @@ PHC_HD dd dd_sincos_kernel(dd r, int which) {
 }
 
 // (cos theta, sin theta), each correctly rounded.
-PHC_HD cx cr_cossin(double theta) {
+PHC_HD_COLD cx cr_cossin(double theta) {
   // pi/2 = P1 + P2 + P3 (53 + 53 + 53 bits)
   const double P1 = 0x1.921fb54442d18p+0;
   const double P2 = 0x1.1a62633145c07p-54;

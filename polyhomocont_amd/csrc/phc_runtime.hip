@@ -196,24 +196,37 @@ int get_trig(phc_ctx* ctx, int nsamples, cx** out) {
 }
 
 template <int G>
-hipError_t launch_track(const Launch& L, int blocks, int threads,
-                        size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL(track_kernel<G>, dim3(blocks), dim3(threads), lds, s, L);
+hipError_t launch_track(const Launch& L, bool staged, int blocks,
+                        int threads, size_t lds, hipStream_t s) {
+  if (staged)
+    hipLaunchKernelGGL((track_kernel<G, true>), dim3(blocks), dim3(threads),
+                       lds, s, L);
+  else
+    hipLaunchKernelGGL((track_kernel<G, false>), dim3(blocks), dim3(threads),
+                       lds, s, L);
   return hipGetLastError();
 }
 
-hipError_t launch_g(int G, const Launch& L, int blocks, int threads,
-                    size_t lds, hipStream_t s) {
+hipError_t launch_g(int G, const Launch& L, bool staged, int blocks,
+                    int threads, size_t lds, hipStream_t s) {
   switch (G) {
 #ifndef PHC_ONLY_G8
     case 4:
-      return launch_track<4>(L, blocks, threads, lds, s);
+      return launch_track<4>(L, staged, blocks, threads, lds, s);
     case 16:
-      return launch_track<16>(L, blocks, threads, lds, s);
+      return launch_track<16>(L, staged, blocks, threads, lds, s);
 #endif
     default:
-      return launch_track<8>(L, blocks, threads, lds, s);
+      return launch_track<8>(L, staged, blocks, threads, lds, s);
   }
+}
+
+// Largest LDS a staged workgroup may use (leaves room for two per CU).
+constexpr size_t STAGED_LDS_LIMIT = 64 * 1024;
+
+size_t group_lds_bytes(int G, int expmax) {
+  return ((size_t)(expmax + 1) * G + (size_t)G * G + G) * sizeof(cx) +
+         (size_t)G * sizeof(int);
 }
 
 // A prepared multi-request run for one lanes class, with device buffers.
@@ -230,6 +243,7 @@ struct ClassRun {
   int expmax = 1;
   int blocks = 0, threads = 0;
   size_t lds = 0;
+  bool staged = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -358,7 +372,15 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
     blocks = std::min<int64_t>(blocks, cus * 16 * (WG / threads));
     cr.blocks = (int)std::max<int64_t>(blocks, 1);
     cr.threads = threads;
-    cr.lds = (size_t)per_wg * (cr.expmax + 1) * G * sizeof(cx);
+    cr.lds = (size_t)per_wg * group_lds_bytes(G, cr.expmax);
+    if (cr.reqs.size() == 1) {
+      const size_t prog = staged_req_bytes() +
+                          cr.reqs[0]->terms.size() * sizeof(TermRec);
+      if (cr.lds + prog <= STAGED_LDS_LIMIT) {
+        cr.staged = true;
+        cr.lds += prog;
+      }
+    }
     cr.d_slots = (PathState*)arena_take(
         st->arena, (size_t)cr.blocks * per_wg * G * sizeof(PathState));
     st->classes.push_back(cr);
@@ -389,7 +411,8 @@ int run_classes(phc_ctx* ctx, phc_staged* st) {
     if (!cr.ev0) HIPCHK(ctx, hipEventCreate(&cr.ev0));
     if (!cr.ev1) HIPCHK(ctx, hipEventCreate(&cr.ev1));
     HIPCHK(ctx, hipEventRecord(cr.ev0, ctx->stream));
-    HIPCHK(ctx, launch_g(cr.G, L, cr.blocks, cr.threads, cr.lds, ctx->stream));
+    HIPCHK(ctx, launch_g(cr.G, L, cr.staged, cr.blocks, cr.threads, cr.lds,
+                         ctx->stream));
     HIPCHK(ctx, hipEventRecord(cr.ev1, ctx->stream));
   }
   return PHC_OK;
@@ -579,6 +602,7 @@ int phc_stage_launch_info(phc_ctx* ctx, phc_staged* st, int32_t* info) {
   info[1] = cr.blocks;
   info[2] = cr.threads;
   info[3] = (int32_t)cr.lds;
+  info[4] = cr.staged ? 1 : 0;
   return PHC_OK;
 }
 

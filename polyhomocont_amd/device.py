@@ -153,12 +153,13 @@ class Staged:
         return float(ms[0])
 
     def launch_info(self):
-        info = np.zeros(4, np.int32)
+        info = np.zeros(8, np.int32)
         self.ctx._check(self.ctx.L.phc_stage_launch_info(self.ctx.h, self.h,
                                                          info.ctypes.data),
                         "phc_stage_launch_info")
         return dict(lanes_per_path=int(info[0]), workgroups=int(info[1]),
-                    threads=int(info[2]), lds_bytes=int(info[3]))
+                    threads=int(info[2]), lds_bytes=int(info[3]),
+                    staged=bool(info[4]))
 
     def close(self):
         if self.h:
