@@ -117,13 +117,23 @@ struct PathState {
   cx est, prev, zold, resz, zprev, zsum;
 };
 
+// A queued path of the serving engine: request slot and path index.
+struct QItem {
+  int32_t req, path;
+};
+
 struct Launch {
   const Req* reqs;
   const int64_t* req_base;  // [nreq + 1] prefix sums of path counts
   int32_t nreq;
   int32_t expmax;           // largest maxexp of the requests
   unsigned long long* next; // work-queue head
-  int64_t total;            // paths in the queue
+  int64_t total;            // paths in the queue (fixed-size runs)
+  // serving: items[i % cap] for head <= i < *tail, appended by the host
+  // between launches
+  const QItem* items;
+  unsigned long long cap;
+  const unsigned long long* tail;
   PathState* slots;         // [groups][G] saved state (resumable launches)
   int32_t resume;           // load saved states at entry
   int32_t max_units;        // units per group per launch, 0 = unlimited
@@ -962,23 +972,44 @@ __global__ void __launch_bounds__(WG)
   unsigned int* fb = L.counters;
   for (;;) {
     if (S.pc == PC_IDLE) {
-      unsigned long long idx = 0;
-      if (lane == 0) idx = atomicAdd(L.next, 1ull);
-      idx = __shfl((long long)idx, 0, G);
-      if ((long long)idx >= L.total) break;
-      int lo = 0;
-      if (!STAGED) {
-        int hi = L.nreq - 1;
-        while (lo < hi) {  // last request with base <= idx
-          const int mid = (lo + hi + 1) >> 1;
-          if (L.req_base[mid] <= (long long)idx)
-            lo = mid;
-          else
-            hi = mid - 1;
+      if (L.max_units && units >= L.max_units) break;
+      // pop the next queue index; -1 when the queue is empty
+      long long idx = -1;
+      if (lane == 0) {
+        const unsigned long long end =
+            L.items ? __atomic_load_n(L.tail, __ATOMIC_RELAXED)
+                    : (unsigned long long)L.total;
+        unsigned long long h = __atomic_load_n(L.next, __ATOMIC_RELAXED);
+        while (h < end) {
+          const unsigned long long old = atomicCAS(L.next, h, h + 1ull);
+          if (old == h) {
+            idx = (long long)h;
+            break;
+          }
+          h = old;
         }
       }
+      idx = __shfl(idx, 0, G);
+      if (idx < 0) break;
+      int lo = 0;
+      if (L.items) {
+        const QItem it = L.items[(unsigned long long)idx % L.cap];
+        lo = it.req;
+        S.path = it.path;
+      } else {
+        if (!STAGED) {
+          int hi = L.nreq - 1;
+          while (lo < hi) {  // last request with base <= idx
+            const int mid = (lo + hi + 1) >> 1;
+            if (L.req_base[mid] <= idx)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+        }
+        S.path = (int32_t)(idx - L.req_base[lo]);
+      }
       S.req = lo;
-      S.path = (int32_t)((long long)idx - L.req_base[lo]);
       const Req& R = STAGED ? *sreq : L.reqs[lo];
       Machine<G> M{R, S, lane, fb};
       M.start_path((lane < R.N) ? R.start[(size_t)S.path * R.N + lane]

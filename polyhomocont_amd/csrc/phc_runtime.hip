@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -655,3 +656,5 @@ int phc_track_batch(phc_ctx* ctx, int32_t nsys, const phc_system* sys,
 }
 
 }  // extern "C"
+
+#include "phc_server.inc"

@@ -47,6 +47,12 @@ def lib():
         L.phc_stage_elapsed.argtypes = [P, P, P]
         L.phc_stage_launch_info.argtypes = [P, P, P]
         L.phc_track_batch.argtypes = [P, i32, P, P, P, P, P, P]
+        L.phc_server_create.argtypes = [P, P, C.POINTER(P)]
+        L.phc_server_submit.argtypes = [P, P, P, P, i64, P, P, P]
+        L.phc_server_progress.argtypes = [P]
+        L.phc_server_poll.argtypes = [P, C.c_uint64]
+        L.phc_server_wait.argtypes = [P, C.c_uint64]
+        L.phc_server_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -164,4 +170,54 @@ class Staged:
     def close(self):
         if self.h:
             self.ctx.L.phc_stage_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class Server:
+    """Continuous-batching serving engine (phc_server_* of include/phc.h)."""
+
+    def __init__(self, ctx, max_resident_paths=0, hbm_budget_bytes=0,
+                 units_per_launch=0):
+        self.ctx = ctx
+        cfg = abi.ServerConfig(max_resident_paths, hbm_budget_bytes,
+                               units_per_launch, 0)
+        h = C.c_void_p()
+        ctx._check(ctx.L.phc_server_create(ctx.h, C.byref(cfg), C.byref(h)),
+                   "phc_server_create")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, packed, hom, opts, start):
+        """Queues a request; returns (ticket, EndpointArrays filled on
+        completion)."""
+        start = np.ascontiguousarray(start, np.complex128)
+        out = abi.EndpointArrays(start.shape[0], packed.n + 1)
+        t = C.c_uint64()
+        self.ctx._check(self.ctx.L.phc_server_submit(
+            self.h, C.byref(packed.c), C.byref(hom.c), C.byref(opts),
+            start.shape[0], start.ctypes.data, C.byref(out.c), C.byref(t)),
+            "phc_server_submit")
+        self._keep[t.value] = (packed, hom, opts, start, out)
+        return t.value, out
+
+    def progress(self):
+        rc = self.ctx.L.phc_server_progress(self.h)
+        if rc < 0:
+            self.ctx._check(rc, "phc_server_progress")
+        return rc
+
+    def done(self, ticket):
+        return self.ctx.L.phc_server_poll(self.h, ticket) == 0
+
+    def wait(self, ticket):
+        self.ctx._check(self.ctx.L.phc_server_wait(self.h, ticket),
+                        "phc_server_wait")
+        return self._keep[ticket][4]
+
+    def release(self, ticket):
+        self._keep.pop(ticket, None)
+
+    def close(self):
+        if self.h:
+            self.ctx.L.phc_server_destroy(self.h)
             self.h = None
