@@ -34,6 +34,12 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix), dense
 PEAK_HBM_GBS = 8000.0
 
 
+def log(msg):
+    """Progress on stderr (the JSON result line goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr,
+          flush=True)
+
+
 def load_traffic(name):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
     p = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
@@ -66,6 +72,108 @@ def cpu_baseline(usys, opts, npaths, budget_s):
                        f"(1 host core, {dt:.1f} s)"), rec, count
 
 
+def percentiles(xs):
+    import numpy as np
+    xs = np.asarray(xs) * 1e3
+    return {"p50": round(float(np.percentile(xs, 50)), 3),
+            "p99": round(float(np.percentile(xs, 99)), 3),
+            "mean": round(float(xs.mean()), 3)}
+
+
+def serving_benchmark(ctx, host_lib, count):
+    """The mixed request stream served by (a) the plain per-solve tracker,
+    one request at a time, and (b) the continuous-batching engine:
+    saturation throughput with every request queued at t = 0, then
+    per-request latency (arrival -> endpoint records on the host) under
+    Poisson arrivals at 70% of the plain tracker's saturation rate."""
+    import numpy as np
+    from polyhomocont_amd import abi, device, workloads
+
+    systems = workloads.serving_stream(count)
+    reqs = []
+    for usys in systems:
+        opts = abi.default_options()
+        g, b, a, ns = host_lib.homotopy(usys, opts.seed)
+        reqs.append((host_lib.pack(usys), abi.HomotopyData(g, b, a), opts,
+                     host_lib.start_points(usys, opts.seed)))
+    paths = sum(len(r[3]) for r in reqs)
+    log(f"serving: {len(reqs)} requests, {paths} paths prepared")
+    # warm both paths (code objects, trig tables)
+    ctx.track_paths(*reqs[0])
+    srv = device.Server(ctx, max_resident_paths=16384,
+                        hbm_budget_bytes=2 << 30, units_per_launch=64)
+    t, _ = srv.submit(*reqs[0])
+    srv.wait(t)
+
+    t0 = time.perf_counter()
+    plain_out = [ctx.track_paths(*r) for r in reqs]
+    plain_s = time.perf_counter() - t0
+    log(f"serving: plain tracker saturation {plain_s:.2f} s")
+
+    t0 = time.perf_counter()
+    tickets = [srv.submit(*r)[0] for r in reqs]
+    for t in tickets:
+        srv.wait(t)
+    serve_s = time.perf_counter() - t0
+    log(f"serving: engine saturation {serve_s:.2f} s")
+    same = all(np.array_equal(srv._keep[t][4].z, p.z) and
+               np.array_equal(srv._keep[t][4].status, p.status)
+               for t, p in zip(tickets, plain_out))
+    for t in tickets:
+        srv.release(t)
+
+    lam = 0.7 * len(reqs) / plain_s
+    rng = np.random.default_rng(105)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
+
+    # plain tracker: one FIFO server
+    lat_plain = []
+    t0 = time.perf_counter()
+    for r, a in zip(reqs, arrivals):
+        now = time.perf_counter() - t0
+        if now < a:
+            time.sleep(a - now)
+        ctx.track_paths(*r)
+        lat_plain.append(time.perf_counter() - t0 - a)
+    log("serving: plain tracker latency run done")
+
+    # serving engine: submit on arrival, progress continuously
+    lat_serve = [None] * len(reqs)
+    pending = {}
+    nxt = 0
+    t0 = time.perf_counter()
+    while nxt < len(reqs) or pending:
+        now = time.perf_counter() - t0
+        if now > arrivals[-1] + 120.0:
+            raise RuntimeError("serving latency run did not finish")
+        while nxt < len(reqs) and arrivals[nxt] <= now:
+            pending[srv.submit(*reqs[nxt])[0]] = nxt
+            nxt += 1
+        if not pending:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+            continue
+        srv.progress()
+        for t in [t for t in pending if srv.done(t)]:
+            i = pending.pop(t)
+            lat_serve[i] = time.perf_counter() - t0 - arrivals[i]
+            srv.release(t)
+    srv.close()
+    log("serving: engine latency run done")
+    return {
+        "workload": f"{len(reqs)} requests, alternately random sparse "
+                    "(n 3..8, degree 2..4, 5..40 terms) and dense (n 2..7, "
+                    "degree 2..3), 8..4096 paths each",
+        "paths": paths,
+        "plain_paths_per_s": round(paths / plain_s, 1),
+        "serving_paths_per_s": round(paths / serve_s, 1),
+        "serving_speedup": round(plain_s / serve_s, 2),
+        "records_identical": bool(same),
+        "qps": round(lam, 3),
+        "latency_ms": {"serving": percentiles(lat_serve),
+                       "plain": percentiles(lat_plain)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +182,8 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; the path "
@@ -96,6 +206,7 @@ def main():
 
     ctx = device.Context(0)
     st = device.Staged(ctx, packed, hom, opts, start)
+    log(f"staged {args.workload}: {npaths} paths, launch {st.launch_info()}")
     for _ in range(args.warmup):
         st.run()
         st.sync()
@@ -110,6 +221,7 @@ def main():
     st.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    log(f"timed {args.steps} steps: {(t1 - t0) / args.steps * 1e3:.2f} ms/step")
     wall = t1 - t0
     rec = st.fetch()
     info = st.launch_info()
@@ -161,6 +273,7 @@ def main():
     }
 
     if not args.no_cpu:
+        log("cpu baseline")
         cb, want, cnt = cpu_baseline(usys, opts, npaths, args.cpu_seconds)
         out["cpu_baseline"] = cb
         same = all(np.array_equal(getattr(rec, f)[:cnt], getattr(want, f)[:cnt])
@@ -172,6 +285,8 @@ def main():
                          "classification_equal": bool(same),
                          "bit_identical_fraction": bit}
         out["gpu_vs_cpu_core"] = round(value / cb["value"], 1)
+    if not args.no_serving:
+        out["serving"] = serving_benchmark(ctx, H, args.serving_requests)
     print(json.dumps(out), flush=True)
 
 

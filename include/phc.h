@@ -185,6 +185,14 @@ int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
  * out[2], out[3] reserved (0). */
 int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
 
+/* Lane mapping of later tracking calls: 0 chooses per launch (one path
+ * per G-lane group, or one path per wavefront with the terms of each
+ * equation split across 64/G replicas when the former would leave most
+ * SIMDs idle), 1 forces one path per group, 2 one path per wavefront where
+ * N <= 64/G. wg_threads: 0 (chosen per launch), 64, 128 or 256. Results do
+ * not depend on the mapping. */
+int phc_ctx_set_mapping(phc_ctx *ctx, int mapping, int wg_threads);
+
 /* Last error message of the context (static storage, never NULL). */
 const char *phc_ctx_last_error(const phc_ctx *ctx);
 
@@ -213,7 +221,8 @@ int phc_stage_destroy(phc_ctx *ctx, phc_staged *st);
 int phc_stage_elapsed(phc_ctx *ctx, phc_staged *st, double *ms);
 /* Launch shape of the first lanes class: info[0] lanes per path, [1]
  * workgroups, [2] threads per workgroup, [3] LDS bytes per workgroup,
- * [4] 1 if the system program is staged in LDS. */
+ * [4] 1 if the system program is staged in LDS, [5] 1 if one path runs
+ * per wavefront (split evaluation). */
 int phc_stage_launch_info(phc_ctx *ctx, phc_staged *st, int32_t *info);
 int phc_ctx_sync(phc_ctx *ctx);
 

@@ -58,6 +58,7 @@ struct TermRec {
 // output arrays (all device pointers).
 struct alignas(16) Req {
   int32_t n, N, maxexp, nterms;
+  int32_t maxterms, reserved0, reserved1, reserved2;  // largest equation
   int32_t first[PHC_MAX_UNKNOWNS + 1];
   int32_t deg[PHC_MAX_UNKNOWNS];
   cx gamma;
@@ -272,14 +273,95 @@ struct GroupLds {
   }
 };
 
+// Split mode: one path per wavefront, R = 64 / G replicas of the G-lane
+// group. Per chunk c, replica g evaluates term c R + g of every equation
+// and writes its value and derivatives to CB; role lanes then add them in
+// term order: lane (g, r < n) owns dF_r/dz_g, lane (g, n) owns F_g.
+template <int G>
+struct SplitLds {
+  cx* CB;  // [R][G][G + 1]: value, then derivatives by nonzero position
+  cx* AE;  // [G][G] exchanged Jacobian rows
+  cx* FE;  // [G] exchanged values
+  static constexpr int R = 64 / G;
+  PHC_D static size_t bytes() {
+    return ((size_t)R * G * (G + 1) + (size_t)G * G + G) * sizeof(cx);
+  }
+  PHC_D void bind(char* base) {
+    CB = (cx*)base;
+    AE = CB + (size_t)R * G * (G + 1);
+    FE = AE + (size_t)G * G;
+  }
+};
+
+// Mask of the unknowns with nonzero exponent in a term (bit j = unknown j),
+// stored by the host in the top 16 bits of e[3] when N <= 8.
+PHC_D unsigned term_mask(const TermRec& t) { return t.e[3] >> 16; }
+
+// Products of one term (src/sparse.f90:261-276): the value c * prod and,
+// for every unknown j with a nonzero exponent, c e_j v_j^(e_j - 1) times
+// the prefix and suffix products of the other factors. Without SPLIT the
+// derivatives are added to the row accumulator hz[j]; with SPLIT the a-th
+// derivative (a-th nonzero exponent) is stored in cb[1 + a].
+template <int G, bool SPLIT>
+PHC_D void term_products(const TermRec& tr, const int N,
+                         const cx* __restrict__ T, cx (&hz)[G], cx& val,
+                         cx* __restrict__ cb) {
+  const cx c = mk(tr.re, tr.im);
+  int ex[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) ex[j] = (tr.e[j >> 2] >> (8 * (j & 3))) & 255;
+  if (G > 8) {  // bytes past the last unknown may hold the term mask
+#pragma unroll
+    for (int j = 8; j < G; ++j) ex[j] = j < N ? ex[j] : 0;
+  }
+  // suffix products over the nonzero exponents
+  cx suf[G];
+  bool sfs[G];
+  cx S = mk(1.0, 0.0);
+  bool sset = false;
+#pragma unroll
+  for (int j = G - 1; j >= 0; --j) {
+    sfs[j] = sset;
+    suf[j] = S;
+    if (ex[j] > 0) {
+      const cx pv = T[ex[j] * G + j];
+      S = sset ? cmul(pv, S) : pv;
+      sset = true;
+    }
+  }
+  // prefix products, derivatives, value
+  cx P = mk(1.0, 0.0);
+  bool pset = false;
+  int a = 1;
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    if (ex[j] > 0) {
+      cx d = cscale((double)ex[j], c);
+      if (ex[j] >= 2) d = cmul(d, T[(ex[j] - 1) * G + j]);
+      if (pset) d = cmul(d, P);
+      if (sfs[j]) d = cmul(d, suf[j]);
+      if (SPLIT) {
+        cb[a] = d;
+        ++a;
+      } else {
+        hz[j] = cadd(hz[j], d);
+      }
+      const cx pv = T[ex[j] * G + j];
+      P = pset ? cmul(P, pv) : pv;
+      pset = true;
+    }
+  }
+  val = cmul(c, P);
+}
+
 // Evaluates H, H_z, H_t at (zq, t) and, for PC_TAN / PC_NEWT, solves
 // H_z x = -(H_t dt) / -H. Returns x_r in lane r. For PC_BE returns the
 // backward error of src/endgame.f90:269-285 in *be.
-template <int G>
+template <int G, bool SPLIT>
 PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
-                const int lane, const int mode, const cx zq, const cx t,
-                const cx dt, const GroupLds<G>& L, cx& xout, int& okout,
-                double& be) {
+                const int lane, const int rep, const int mode, const cx zq,
+                const cx t, const cx dt, const GroupLds<G>& L,
+                const SplitLds<G>& SL, cx& xout, int& okout, double& be) {
   const int N = R.N, n = R.n;
   const bool row = lane < N;
   cx* __restrict__ T = L.T;
@@ -301,52 +383,73 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
   cx hz[G];
 #pragma unroll
   for (int j = 0; j < G; ++j) hz[j] = mk(0.0, 0.0);
-  int k0 = 0, k1 = 0;
-  if (lane < n) {
-    k0 = R.first[lane];
-    k1 = R.first[lane + 1];
-  }
-  TermRec nxt;
-  if (k0 < k1) nxt = terms[k0];
-  for (int k = k0; k < k1; ++k) {
-    const TermRec tr = nxt;
-    if (k + 1 < k1) nxt = terms[k + 1];  // prefetch the next term
-    const cx c = mk(tr.re, tr.im);
-    int ex[G];
-#pragma unroll
-    for (int j = 0; j < G; ++j) ex[j] = (tr.e[j >> 2] >> (8 * (j & 3))) & 255;
-    // suffix products over the nonzero exponents
-    cx suf[G];
-    bool sfs[G];
-    cx S = mk(1.0, 0.0);
-    bool sset = false;
-#pragma unroll
-    for (int j = G - 1; j >= 0; --j) {
-      sfs[j] = sset;
-      suf[j] = S;
-      if (ex[j] > 0) {
-        const cx pv = T[ex[j] * G + j];
-        S = sset ? cmul(pv, S) : pv;
-        sset = true;
-      }
+  if (!SPLIT) {
+    int k0 = 0, k1 = 0;
+    if (lane < n) {
+      k0 = R.first[lane];
+      k1 = R.first[lane + 1];
     }
-    // prefix products, derivatives, value
-    cx P = mk(1.0, 0.0);
-    bool pset = false;
-#pragma unroll
-    for (int j = 0; j < G; ++j) {
-      if (ex[j] > 0) {
-        cx d = cscale((double)ex[j], c);
-        if (ex[j] >= 2) d = cmul(d, T[(ex[j] - 1) * G + j]);
-        if (pset) d = cmul(d, P);
-        if (sfs[j]) d = cmul(d, suf[j]);
-        hz[j] = cadd(hz[j], d);
-        const cx pv = T[ex[j] * G + j];
-        P = pset ? cmul(P, pv) : pv;
-        pset = true;
-      }
+    TermRec nxt;
+    if (k0 < k1) nxt = terms[k0];
+    for (int k = k0; k < k1; ++k) {
+      const TermRec tr = nxt;
+      if (k + 1 < k1) nxt = terms[k + 1];  // prefetch the next term
+      cx val;
+      term_products<G, false>(tr, N, T, hz, val, nullptr);
+      f = cadd(f, val);
     }
-    f = cadd(f, cmul(c, P));
+  } else {
+    // chunks of R terms per equation; replica `rep` takes term c R + rep
+    constexpr int RR = SplitLds<G>::R;
+    const int nt_row = (lane < n) ? R.first[lane + 1] - R.first[lane] : 0;
+    const int row_first = (lane < n) ? R.first[lane] : 0;
+    cx* const cb_mine = SL.CB + ((size_t)rep * G + lane) * (G + 1);
+    // role of this lane in the ordered sums
+    const bool jac_role = lane < n && rep < N;
+    const bool f_role = lane == n && rep < n;
+    const int f_first = f_role ? R.first[rep] : 0;
+    const int f_nt = f_role ? R.first[rep + 1] - R.first[rep] : 0;
+    const unsigned below = (1u << rep) - 1u;
+    cx acc = mk(0.0, 0.0);
+    bool aset = false;
+    const int nchunks = (R.maxterms + RR - 1) / RR;
+    for (int c = 0; c < nchunks; ++c) {
+      const int m = c * RR + rep;
+      if (lane < n && m < nt_row) {
+        cx val;
+        term_products<G, true>(terms[row_first + m], N, T, hz, val, cb_mine);
+        cb_mine[0] = val;
+      }
+      wave_lds_fence();
+#pragma unroll
+      for (int t = 0; t < RR; ++t) {
+        const int mt = c * RR + t;
+        if (jac_role && mt < nt_row) {
+          const unsigned mask = term_mask(terms[row_first + mt]);
+          if ((mask >> rep) & 1u) {
+            const int a = __popc(mask & below);
+            const cx v = SL.CB[((size_t)t * G + lane) * (G + 1) + 1 + a];
+            acc = aset ? cadd(acc, v) : v;
+            aset = true;
+          }
+        }
+        if (f_role && mt < f_nt) {
+          const cx v = SL.CB[((size_t)t * G + rep) * (G + 1)];
+          acc = aset ? cadd(acc, v) : v;
+          aset = true;
+        }
+      }
+      wave_lds_fence();
+    }
+    if (jac_role) SL.AE[lane * G + rep] = acc;
+    if (f_role) SL.FE[rep] = acc;
+    wave_lds_fence();
+    if (lane < n) {
+#pragma unroll
+      for (int j = 0; j < G; ++j)
+        if (j < N) hz[j] = SL.AE[lane * G + j];
+      f = SL.FE[lane];
+    }
   }
 
   // 3. homotopy rows (src/homotopy.f90:491-511)
@@ -512,6 +615,7 @@ struct Machine {
   PathState& S;
   const int lane;
   unsigned int* fb;
+  bool writer;  // the replica that stores results (split mode: replica 0)
 
   PHC_D double norm(cx v) const { return gnorm<G>(v, lane, R.N); }
 
@@ -882,8 +986,8 @@ struct Machine {
   // --- results -------------------------------------------------------------
   PHC_D void finish(cx z, int wind, int egs, double egerr, double egres) {
     const int64_t k = S.path;
-    if (lane < R.N) R.o_z[k * R.N + lane] = z;
-    if (lane == 0) {
+    if (writer && lane < R.N) R.o_z[k * R.N + lane] = z;
+    if (writer && lane == 0) {
       R.o_status[k] = S.pstatus;
       R.o_t[k] = S.pt;
       R.o_wind[k] = wind;
@@ -933,19 +1037,31 @@ struct Machine {
 // program are copied into LDS once per workgroup and every group reads
 // them from there. Otherwise requests and programs are read from global
 // memory (L2).
+// SPLIT: one path per wavefront (R = 64/G identical replicas of the group
+// that split the terms of each equation in the evaluation); otherwise one
+// path per G-lane group.
 constexpr size_t staged_req_bytes() { return (sizeof(Req) + 15) & ~size_t(15); }
 
-template <int G, bool STAGED>
+template <int G, bool SPLIT>
+PHC_D size_t unit_lds_bytes(int expmax) {
+  return SPLIT ? GroupLds<G>::bytes(expmax) + SplitLds<G>::bytes()
+               : GroupLds<G>::bytes(expmax);
+}
+
+template <int G, bool STAGED, bool SPLIT>
 __global__ void __launch_bounds__(WG)
     track_kernel(Launch L) {
   extern __shared__ __align__(16) char lds_raw[];
   const int lane = threadIdx.x % G;
-  const int gi = threadIdx.x / G;
-  const int slot = blockIdx.x * (blockDim.x / G) + gi;
+  const int rep = SPLIT ? (threadIdx.x % 64) / G : 0;
+  const int owner = SPLIT ? (int)(threadIdx.x / 64) : (int)(threadIdx.x / G);
+  const int per_block = SPLIT ? (int)(blockDim.x / 64) : (int)(blockDim.x / G);
+  const int slot = blockIdx.x * per_block + owner;
+  const bool writer = rep == 0;
 
   Req* sreq = (Req*)lds_raw;
   TermRec* sterms = (TermRec*)(lds_raw + staged_req_bytes());
-  size_t group_base = 0;
+  size_t base = 0;
   if (STAGED) {
     const uint4* src = (const uint4*)&L.reqs[0];
     uint4* dst = (uint4*)sreq;
@@ -956,11 +1072,13 @@ __global__ void __launch_bounds__(WG)
     uint4* tdst = (uint4*)sterms;
     for (int i = threadIdx.x; i < nt * 2; i += blockDim.x) tdst[i] = tsrc[i];
     __syncthreads();
-    group_base = staged_req_bytes() + (size_t)nt * sizeof(TermRec);
+    base = staged_req_bytes() + (size_t)nt * sizeof(TermRec);
   }
+  char* mine = lds_raw + base + (size_t)owner * unit_lds_bytes<G, SPLIT>(L.expmax);
   GroupLds<G> gl;
-  gl.bind(lds_raw + group_base + (size_t)gi * GroupLds<G>::bytes(L.expmax),
-          L.expmax);
+  gl.bind(mine, L.expmax);
+  SplitLds<G> sl;
+  if (SPLIT) sl.bind(mine + GroupLds<G>::bytes(L.expmax));
 
   PathState S;
   if (L.resume) {
@@ -975,7 +1093,7 @@ __global__ void __launch_bounds__(WG)
       if (L.max_units && units >= L.max_units) break;
       // pop the next queue index; -1 when the queue is empty
       long long idx = -1;
-      if (lane == 0) {
+      if (SPLIT ? (threadIdx.x % 64 == 0) : (lane == 0)) {
         const unsigned long long end =
             L.items ? __atomic_load_n(L.tail, __ATOMIC_RELAXED)
                     : (unsigned long long)L.total;
@@ -989,7 +1107,7 @@ __global__ void __launch_bounds__(WG)
           h = old;
         }
       }
-      idx = __shfl(idx, 0, G);
+      idx = SPLIT ? __shfl(idx, 0, 64) : __shfl(idx, 0, G);
       if (idx < 0) break;
       int lo = 0;
       if (L.items) {
@@ -1011,13 +1129,13 @@ __global__ void __launch_bounds__(WG)
       }
       S.req = lo;
       const Req& R = STAGED ? *sreq : L.reqs[lo];
-      Machine<G> M{R, S, lane, fb};
+      Machine<G> M{R, S, lane, fb, writer};
       M.start_path((lane < R.N) ? R.start[(size_t)S.path * R.N + lane]
                                 : mk(0.0, 0.0));
     } else {
       if (L.max_units && units >= L.max_units) break;
       const Req& R = STAGED ? *sreq : L.reqs[S.req];
-      Machine<G> M{R, S, lane, fb};
+      Machine<G> M{R, S, lane, fb, writer};
       cx t = mk(0.0, 0.0), dt = mk(0.0, 0.0);
       if (S.pc != PC_BE) {
         const double sv = (S.pc == PC_TAN) ? S.se : S.sn;
@@ -1030,8 +1148,9 @@ __global__ void __launch_bounds__(WG)
       cx x;
       int ok;
       double be = 0.0;
-      unit<G>(R, STAGED ? sterms : R.terms, lane, S.pc,
-              (lane < R.N) ? S.zq : mk(0.0, 0.0), t, dt, gl, x, ok, be);
+      unit<G, SPLIT>(R, STAGED ? sterms : R.terms, lane, rep, S.pc,
+                     (lane < R.N) ? S.zq : mk(0.0, 0.0), t, dt, gl, sl, x, ok,
+                     be);
       units++;
       if (S.pc == PC_TAN)
         M.on_tangent(x, ok);
@@ -1041,12 +1160,12 @@ __global__ void __launch_bounds__(WG)
         M.on_backward_error(be);
     }
     const Req& R = STAGED ? *sreq : L.reqs[S.req];
-    Machine<G> M{R, S, lane, fb};
+    Machine<G> M{R, S, lane, fb, writer};
     M.run();
   }
   if (L.slots) L.slots[(size_t)slot * G + lane] = S;
-  if (S.pc != PC_IDLE && lane == 0 && L.active) atomicAdd(L.active, 1u);
-  if (lane == 0 && L.counters) atomicAdd(&L.counters[1], (unsigned)units);
+  if (writer && S.pc != PC_IDLE && lane == 0 && L.active) atomicAdd(L.active, 1u);
+  if (writer && lane == 0 && L.counters) atomicAdd(&L.counters[1], (unsigned)units);
 }
 
 }  // namespace phc

@@ -31,6 +31,7 @@ struct phc_ctx {
   int num_cus = 0;
   int lanes_override[PHC_MAX_UNKNOWNS + 1] = {0};
   int wg_threads = 0;  // 0: choose per launch
+  int mapping = 0;     // 0: by rule, 1: one path per group, 2: per wavefront
   std::map<int, cx*> trig;  // nsamples -> device table
   unsigned long long counters[4] = {0, 0, 0, 0};  // of the last run
   std::mutex mu;
@@ -131,6 +132,20 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
     (void)tot;
   }
   r.maxexp = std::max(maxexp, 1);
+  // Nonzero-exponent mask of each term in e[3] bits 16..31. Exponent bytes
+  // of unknowns 0..7 live in e[0..1], so e[3] is free only when N <= 8; the
+  // decode of wider groups ignores bytes at j >= N.
+  if (N <= 8) {
+    for (TermRec& t : h.terms) {
+      uint32_t mask = 0;
+      for (int j = 0; j < N; ++j)
+        if ((t.e[j >> 2] >> (8 * (j & 3))) & 255u) mask |= 1u << j;
+      t.e[3] |= mask << 16;
+    }
+  }
+  r.maxterms = 0;
+  for (int i = 0; i < n; ++i)
+    r.maxterms = std::max(r.maxterms, sys->first[i + 1] - sys->first[i]);
   r.gamma = mk(hom->gamma[0], hom->gamma[1]);
   for (int i = 0; i < n; ++i) r.b[i] = mk(hom->b[2 * i], hom->b[2 * i + 1]);
   for (int j = 0; j < N; ++j)
@@ -196,29 +211,32 @@ int get_trig(phc_ctx* ctx, int nsamples, cx** out) {
   return PHC_OK;
 }
 
-template <int G>
+template <int G, bool SPLIT>
 hipError_t launch_track(const Launch& L, bool staged, int blocks,
                         int threads, size_t lds, hipStream_t s) {
   if (staged)
-    hipLaunchKernelGGL((track_kernel<G, true>), dim3(blocks), dim3(threads),
-                       lds, s, L);
+    hipLaunchKernelGGL((track_kernel<G, true, SPLIT>), dim3(blocks),
+                       dim3(threads), lds, s, L);
   else
-    hipLaunchKernelGGL((track_kernel<G, false>), dim3(blocks), dim3(threads),
-                       lds, s, L);
+    hipLaunchKernelGGL((track_kernel<G, false, SPLIT>), dim3(blocks),
+                       dim3(threads), lds, s, L);
   return hipGetLastError();
 }
 
 hipError_t launch_g(int G, const Launch& L, bool staged, int blocks,
-                    int threads, size_t lds, hipStream_t s) {
+                    int threads, size_t lds, hipStream_t s,
+                    bool split = false) {
   switch (G) {
 #ifndef PHC_ONLY_G8
     case 4:
-      return launch_track<4>(L, staged, blocks, threads, lds, s);
+      return split ? launch_track<4, true>(L, staged, blocks, threads, lds, s)
+                   : launch_track<4, false>(L, staged, blocks, threads, lds, s);
     case 16:
-      return launch_track<16>(L, staged, blocks, threads, lds, s);
+      return launch_track<16, false>(L, staged, blocks, threads, lds, s);
 #endif
     default:
-      return launch_track<8>(L, staged, blocks, threads, lds, s);
+      return split ? launch_track<8, true>(L, staged, blocks, threads, lds, s)
+                   : launch_track<8, false>(L, staged, blocks, threads, lds, s);
   }
 }
 
@@ -228,6 +246,10 @@ constexpr size_t STAGED_LDS_LIMIT = 64 * 1024;
 size_t group_lds_bytes(int G, int expmax) {
   return ((size_t)(expmax + 1) * G + (size_t)G * G + G) * sizeof(cx) +
          (size_t)G * sizeof(int);
+}
+size_t split_lds_bytes(int G) {
+  const size_t R = 64 / G;
+  return (R * G * (G + 1) + (size_t)G * G + G) * sizeof(cx);
 }
 
 // A prepared multi-request run for one lanes class, with device buffers.
@@ -245,6 +267,7 @@ struct ClassRun {
   int blocks = 0, threads = 0;
   size_t lds = 0;
   bool staged = false;
+  bool split = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -357,23 +380,33 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
     cr.d_next = (unsigned long long*)arena_take(st->arena, 8);
     cr.d_counters = (unsigned int*)arena_take(st->arena, 16);
     cr.d_active = (unsigned int*)arena_take(st->arena, 4);
-    // Launch shape: enough groups for every path up to a persistent cap;
-    // small runs use single-wave workgroups so they spread over all CUs.
+    // Lane mapping: one path per G-lane group, or -- when that would leave
+    // most SIMDs idle and every system has N <= 64/G -- one path per
+    // wavefront with the terms split over 64/G replicas.
     const int G = cr.G;
-    int threads = ctx->wg_threads ? ctx->wg_threads : WG;
-    const int64_t groups_needed = std::max<int64_t>(cr.total, 1);
     const int64_t cus = std::max(ctx->num_cus, 1);
+    bool can_split = G <= 8;
+    for (HostReq* h : cr.reqs) can_split = can_split && h->r.N <= 64 / G;
+    const int64_t row_waves = (std::max<int64_t>(cr.total, 1) * G + 63) / 64;
+    cr.split = can_split && (ctx->mapping == 2 ||
+                             (ctx->mapping == 0 && row_waves < 2 * 4 * cus));
+    const int lanes_per_path = cr.split ? 64 : G;
+    int threads = ctx->wg_threads ? ctx->wg_threads : WG;
+    const int64_t paths_needed = std::max<int64_t>(cr.total, 1);
     if (!ctx->wg_threads) {
       threads = WG;
-      while (threads > 64 && (groups_needed + threads / G - 1) / (threads / G) < 2 * cus)
+      while (threads > 64 &&
+             (paths_needed + threads / lanes_per_path - 1) /
+                     (threads / lanes_per_path) < 2 * cus)
         threads /= 2;
     }
-    const int per_wg = threads / G;
-    int64_t blocks = (groups_needed + per_wg - 1) / per_wg;
+    const int per_wg = threads / lanes_per_path;
+    int64_t blocks = (paths_needed + per_wg - 1) / per_wg;
     blocks = std::min<int64_t>(blocks, cus * 16 * (WG / threads));
     cr.blocks = (int)std::max<int64_t>(blocks, 1);
     cr.threads = threads;
-    cr.lds = (size_t)per_wg * group_lds_bytes(G, cr.expmax);
+    cr.lds = (size_t)per_wg * (group_lds_bytes(G, cr.expmax) +
+                               (cr.split ? split_lds_bytes(G) : 0));
     if (cr.reqs.size() == 1) {
       const size_t prog = staged_req_bytes() +
                           cr.reqs[0]->terms.size() * sizeof(TermRec);
@@ -413,7 +446,7 @@ int run_classes(phc_ctx* ctx, phc_staged* st) {
     if (!cr.ev1) HIPCHK(ctx, hipEventCreate(&cr.ev1));
     HIPCHK(ctx, hipEventRecord(cr.ev0, ctx->stream));
     HIPCHK(ctx, launch_g(cr.G, L, cr.staged, cr.blocks, cr.threads, cr.lds,
-                         ctx->stream));
+                         ctx->stream, cr.split));
     HIPCHK(ctx, hipEventRecord(cr.ev1, ctx->stream));
   }
   return PHC_OK;
@@ -540,6 +573,17 @@ int phc_ctx_counters(const phc_ctx* ctx, uint64_t* out) {
   return PHC_OK;
 }
 
+int phc_ctx_set_mapping(phc_ctx* ctx, int mapping, int wg_threads) {
+  if (!ctx || mapping < 0 || mapping > 2) return PHC_ERR_INVALID_INPUT;
+  if (wg_threads != 0 && wg_threads != 64 && wg_threads != 128 &&
+      wg_threads != 256)
+    return PHC_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->mapping = mapping;
+  ctx->wg_threads = wg_threads;
+  return PHC_OK;
+}
+
 int phc_ctx_sync(phc_ctx* ctx) {
   if (!ctx) return PHC_ERR_INVALID_INPUT;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -604,6 +648,7 @@ int phc_stage_launch_info(phc_ctx* ctx, phc_staged* st, int32_t* info) {
   info[2] = cr.threads;
   info[3] = (int32_t)cr.lds;
   info[4] = cr.staged ? 1 : 0;
+  info[5] = cr.split ? 1 : 0;
   return PHC_OK;
 }
 

@@ -39,6 +39,7 @@ def lib():
         L.phc_ctx_last_error.restype = C.c_char_p
         L.phc_ctx_sync.argtypes = [P]
         L.phc_ctx_counters.argtypes = [P, P]
+        L.phc_ctx_set_mapping.argtypes = [P, C.c_int, C.c_int]
         L.phc_track_paths.argtypes = [P, P, P, P, i64, P, P]
         L.phc_stage_create.argtypes = [P, P, P, P, i64, P, C.POINTER(P)]
         L.phc_stage_run.argtypes = [P, P]
@@ -89,6 +90,12 @@ class Context:
         c = np.zeros(4, np.uint64)
         self.L.phc_ctx_counters(self.h, c.ctypes.data)
         return dict(trig_fallbacks=int(c[0]), units=int(c[1]))
+
+    def set_mapping(self, mapping=0, wg_threads=0):
+        """0: chosen per launch, 1: path per lane group, 2: path per
+        wavefront (split evaluation)."""
+        self._check(self.L.phc_ctx_set_mapping(self.h, mapping, wg_threads),
+                    "phc_ctx_set_mapping")
 
     def lanes_per_path(self, N):
         return self.L.phc_ctx_lanes_per_path(self.h, N)
@@ -165,7 +172,7 @@ class Staged:
                         "phc_stage_launch_info")
         return dict(lanes_per_path=int(info[0]), workgroups=int(info[1]),
                     threads=int(info[2]), lds_bytes=int(info[3]),
-                    staged=bool(info[4]))
+                    staged=bool(info[4]), split=bool(info[5]))
 
     def close(self):
         if self.h:

@@ -230,3 +230,23 @@ def katsura(n):
         es.append(v)
         eqs.append((np.array(cs), np.array(es, np.int32)))
     return UserSystemData(N, eqs)
+
+
+def serving_stream(count=64, seed=105, max_paths=4096, min_paths=8):
+    """Requests of the serving benchmark: alternately a system of the
+    random sparse generator and a dense system (n in 2..7, degree 2..3),
+    keeping those with 8..4096 total-degree paths."""
+    g = Lecuyer(seed)
+    out = []
+    k = 0
+    while len(out) < count:
+        if k % 2 == 0:
+            usys = random_sparse(10_000 + seed * 1000 + k)
+        else:
+            n = g.integer(2, 7)
+            d = g.integer(2, 3)
+            usys = dense_system(n, d, 20_000 + seed * 1000 + k)
+        k += 1
+        if min_paths <= usys.bezout <= max_paths:
+            out.append(usys)
+    return out

@@ -126,3 +126,68 @@ def test_batch_equals_separate_calls(ctx, host):
     outs = ctx.track_batch(items)
     for o, s, it in zip(outs, single, items):
         assert_bit_exact(o, s, len(it[3]))
+
+
+@pytest.mark.parametrize("mapping", [1, 2])
+def test_lane_mappings_bit_exact(ctx, mapping):
+    """One path per lane group and one path per wavefront (terms split
+    across replicas, ordered sums in LDS) give the reference's records."""
+    G = golden()
+    from polyhomocont_amd import host
+    ctx.set_mapping(mapping)
+    try:
+        for name in ("c1", "katsura4", "cyclic5", "mixed", "triple_root",
+                     "infinity"):
+            usys = W.config_system(name) if name == "c1" else W.ref_test_systems()[name]
+            opts = abi.default_options()
+            consts = (G[name + "/gamma"][0], G[name + "/b"], G[name + "/patch"])
+            start = host.library().start_points(usys, opts.seed)
+            print(f"[gpu] mapping {mapping}: {name}", flush=True)
+            got = run_device(ctx, usys, opts, consts, start)
+            n = len(start)
+            for f in FIELDS + ("eg_error", "t"):
+                assert np.array_equal(getattr(got, f)[:n], G[f"{name}/{f}"]), (name, f)
+            assert np.array_equal(got.z[:n], G[name + "/z"]), name
+    finally:
+        ctx.set_mapping(0)
+
+
+def test_c2_split_mapping_matches_reference(ctx, ref):
+    usys = W.config_system("c2")
+    opts = abi.default_options()
+    g, b, a, ns = ref.homotopy(usys, opts.seed)
+    start = ref.start_points(usys, opts.seed, 0, 96)
+    ctx.set_mapping(2)
+    try:
+        got = run_device(ctx, usys, opts, (g, b, a), start)
+    finally:
+        ctx.set_mapping(0)
+    assert_bit_exact(got, ref.endpoints(usys, opts, 0, 96), 96)
+
+
+def _wide_systems():
+    """Systems with 9..12 homogeneous unknowns (16-lane groups): a random
+    sparse system with n = 8 and dense quadrics with n = 9 and n = 11."""
+    sparse8 = next(s for s in (W.random_sparse(104 + k) for k in range(200))
+                   if s.n == 8)
+    return [("sparse n=8", sparse8), ("dense n=9 d=2", W.dense_system(9, 2, 107)),
+            ("dense n=11 d=2", W.dense_system(11, 2, 108))]
+
+
+@pytest.mark.parametrize("mapping", [0, 1])
+def test_wide_systems_bit_exact(ctx, ref, mapping):
+    """16-lane groups decode only the exponent bytes of real unknowns."""
+    opts = abi.default_options()
+    ctx.set_mapping(mapping)
+    try:
+        for name, usys in _wide_systems():
+            g, b, a, ns = ref.homotopy(usys, opts.seed)
+            cnt = min(ns, 24)
+            first = ns // 3
+            start = ref.start_points(usys, opts.seed, first, cnt)
+            print(f"[gpu] wide {name}: paths {first}..{first + cnt} of {ns}",
+                  flush=True)
+            got = run_device(ctx, usys, opts, (g, b, a), start)
+            assert_bit_exact(got, ref.endpoints(usys, opts, first, cnt), cnt)
+    finally:
+        ctx.set_mapping(0)
