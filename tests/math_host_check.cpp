@@ -1,0 +1,115 @@
+// Host build of polyhomocont_amd/csrc/phc_math.h, checked bit for bit
+// against the routines the reference's flang build calls:
+//   cdiv       vs compiler-rt __divdc3 (what flang emits for complex '/')
+//   cabs_      vs glibc hypot (what flang's abs(complex) reaches)
+//   zpowi      vs flang runtime _FortranAzpowi (complex ** integer)
+//   cr_cossin  vs libquadmath cosq/sinq rounded once to double
+// Inputs: random magnitudes over wide binades plus special values (zeros,
+// infinities, NaN, subnormals, values near the hypot scaling thresholds).
+// Prints one line per function: "<name> <checked> <mismatches>".
+#include <float.h>
+#include <math.h>
+#include <quadmath.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "phc_math.h"
+
+using phc::cx;
+
+extern "C" double _Complex __divdc3(double a, double b, double c, double d);
+extern "C" cx _FortranAzpowi(double re, double im, int e);
+
+static uint64_t state = 0x9E3779B97F4A7C15ull;
+static uint64_t next_u64() {
+  state ^= state << 13;
+  state ^= state >> 7;
+  state ^= state << 17;
+  return state;
+}
+static double uniform() { return (double)(next_u64() >> 11) * 0x1p-53; }
+// random sign, mantissa and binade in [-span, span]
+static double wide(int span) {
+  double m = uniform() + 0.5;
+  int e = (int)(next_u64() % (uint64_t)(2 * span + 1)) - span;
+  return (next_u64() & 1 ? -1.0 : 1.0) * ldexp(m, e);
+}
+static const double SPECIAL[] = {0.0, -0.0, 1.0, -1.0, INFINITY, -INFINITY,
+                                 NAN, 0x1p-1074, -0x1p-1070, 0x1p-1022,
+                                 0x1p+511, 0x1.fffffffffffffp+511,
+                                 0x1p+512, 0x1p-511, 0x1.fffffffffffffp-512,
+                                 0x1p+1023, DBL_MAX, 1e-300, 1e300, 3.0};
+static const int NSPECIAL = sizeof(SPECIAL) / sizeof(SPECIAL[0]);
+
+static bool same(double a, double b) {
+  if (isnan(a) && isnan(b)) return true;  // NaN payloads are not compared
+  return memcmp(&a, &b, sizeof(double)) == 0;
+}
+
+int main(int argc, char** argv) {
+  const long n = argc > 1 ? atol(argv[1]) : 1000000;
+
+  // complex division
+  long bad = 0, cnt = 0;
+  for (long i = 0; i < n; ++i) {
+    int span = (i % 4 == 0) ? 1000 : 40;
+    double a = wide(span), b = wide(span), c = wide(span), d = wide(span);
+    cx q = phc::cdiv(phc::mk(a, b), phc::mk(c, d));
+    double _Complex w = __divdc3(a, b, c, d);
+    bad += !(same(q.re, __real__ w) && same(q.im, __imag__ w));
+    ++cnt;
+  }
+  for (int i = 0; i < NSPECIAL; ++i)
+    for (int j = 0; j < NSPECIAL; ++j)
+      for (int k = 0; k < NSPECIAL; k += 3) {
+        double a = SPECIAL[i], b = SPECIAL[k], c = SPECIAL[j], d = SPECIAL[(i + j) % NSPECIAL];
+        cx q = phc::cdiv(phc::mk(a, b), phc::mk(c, d));
+        double _Complex w = __divdc3(a, b, c, d);
+        bad += !(same(q.re, __real__ w) && same(q.im, __imag__ w));
+        ++cnt;
+      }
+  printf("cdiv %ld %ld\n", cnt, bad);
+
+  // absolute value
+  bad = cnt = 0;
+  for (long i = 0; i < n; ++i) {
+    int span = (i % 4 == 0) ? 1070 : 30;
+    double x = wide(span), y = (i % 7 == 0) ? x * (1.0 + wide(60)) : wide(span);
+    bad += !same(phc::cabs_(phc::mk(x, y)), hypot(x, y));
+    ++cnt;
+  }
+  for (int i = 0; i < NSPECIAL; ++i)
+    for (int j = 0; j < NSPECIAL; ++j) {
+      bad += !same(phc::cabs_(phc::mk(SPECIAL[i], SPECIAL[j])),
+                   hypot(SPECIAL[i], SPECIAL[j]));
+      ++cnt;
+    }
+  printf("cabs %ld %ld\n", cnt, bad);
+
+  // complex ** integer
+  bad = cnt = 0;
+  for (long i = 0; i < n / 4; ++i) {
+    double re = wide(8), im = wide(8);
+    int e = (int)(next_u64() % 41) - 20;
+    cx p = phc::zpowi(phc::mk(re, im), e);
+    cx w = _FortranAzpowi(re, im, e);
+    bad += !(same(p.re, w.re) && same(p.im, w.im));
+    ++cnt;
+  }
+  printf("zpowi %ld %ld\n", cnt, bad);
+
+  // correctly rounded cos/sin on the arc-angle range [-8, 8]
+  bad = cnt = 0;
+  for (long i = 0; i < n / 4; ++i) {
+    double th = (uniform() * 2.0 - 1.0) * 8.0;
+    if (i % 16 == 0) th = ldexp(uniform(), -(int)(next_u64() % 60));
+    cx v = phc::cr_cossin(th);
+    bad += !(same(v.re, (double)cosq((__float128)th)) &&
+             same(v.im, (double)sinq((__float128)th)));
+    ++cnt;
+  }
+  printf("cr_cossin %ld %ld\n", cnt, bad);
+  return 0;
+}
