@@ -185,12 +185,13 @@ int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
  * out[2], out[3] reserved (0). */
 int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
 
-/* Lane mapping of later tracking calls: 0 chooses per launch (one path
- * per G-lane group, or one path per wavefront with the terms of each
- * equation split across 64/G replicas when the former would leave most
- * SIMDs idle), 1 forces one path per group, 2 one path per wavefront where
- * N <= 64/G. wg_threads: 0 (chosen per launch), 64, 128 or 256. Results do
- * not depend on the mapping. */
+/* Lane mapping of later tracking calls. 0 chooses per launch: the G*G-lane
+ * grid for N <= 8 (a path's linear system one entry per lane, its
+ * evaluation spread over all lanes), else one path per G-lane group.
+ * 1 forces one path per G-lane group (row per lane), 2 one path per
+ * wavefront with the terms of each equation split across 64/G replicas
+ * where N <= 64/G, 3 the grid where N <= 8. wg_threads: 0 (chosen per
+ * launch), 64, 128 or 256. Results do not depend on the mapping. */
 int phc_ctx_set_mapping(phc_ctx *ctx, int mapping, int wg_threads);
 
 /* Last error message of the context (static storage, never NULL). */
@@ -219,10 +220,12 @@ int phc_stage_destroy(phc_ctx *ctx, phc_staged *st);
  * measured with HIP events recorded on the context stream around each
  * launch; waits for the run to finish. */
 int phc_stage_elapsed(phc_ctx *ctx, phc_staged *st, double *ms);
-/* Launch shape of the first lanes class: info[0] lanes per path, [1]
- * workgroups, [2] threads per workgroup, [3] LDS bytes per workgroup,
- * [4] 1 if the system program is staged in LDS, [5] 1 if one path runs
- * per wavefront (split evaluation). */
+/* Launch shape of the first lanes class (info has room for 8 values):
+ * info[0] group size G (a power of two >= N), [1] workgroups, [2] threads
+ * per workgroup, [3] LDS bytes per workgroup, [4] 1 if the system program
+ * is staged in LDS, [5] lane mapping (0 one path per G-lane group, 1 one
+ * path per wavefront with split evaluation, 2 one path per G*G-lane grid),
+ * [6] lanes per path. */
 int phc_stage_launch_info(phc_ctx *ctx, phc_staged *st, int32_t *info);
 int phc_ctx_sync(phc_ctx *ctx);
 

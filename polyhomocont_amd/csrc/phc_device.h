@@ -58,13 +58,20 @@ struct TermRec {
 // output arrays (all device pointers).
 struct alignas(16) Req {
   int32_t n, N, maxexp, nterms;
-  int32_t maxterms, reserved0, reserved1, reserved2;  // largest equation
+  int32_t maxterms;  // terms of the largest equation
+  int32_t njobs;     // grid unit: job rows (0: no grid tables)
+  int32_t gfold;     // grid unit: max nonzero factors of a term (bits
+                     // 0-7), of a job's prefix (8-15) and suffix (16-23),
+                     // 1 << 24 if some exponent is >= 2
+  int32_t reserved1;
   int32_t first[PHC_MAX_UNKNOWNS + 1];
   int32_t deg[PHC_MAX_UNKNOWNS];
   cx gamma;
   cx b[PHC_MAX_UNKNOWNS];
   cx patch[PHC_MAX_UNKNOWNS];
   const TermRec* terms;
+  const struct WTerm* wterms;  // grid unit: compact terms
+  const uint4* jobs;           // grid unit: [njobs][G G] job records
   const cx* trig;   // [nsamples][TRIG_M + 1]: (cos, sin) of arc angles
   const cx* start;  // [npaths][N]
   int64_t npaths;
@@ -128,6 +135,8 @@ struct Launch {
   const int64_t* req_base;  // [nreq + 1] prefix sums of path counts
   int32_t nreq;
   int32_t expmax;           // largest maxexp of the requests
+  int32_t vmax;             // grid unit: largest term count of the requests
+  int32_t reserved;
   unsigned long long* next; // work-queue head
   int64_t total;            // paths in the queue (fixed-size runs)
   // serving: items[i % cap] for head <= i < *tail, appended by the host
@@ -586,6 +595,10 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
   xout = xmine;
 }
 
+}  // namespace phc
+#include "phc_wave.h"
+namespace phc {
+
 // ------------------------------------------------------------ state machine
 // Internal states: a transition never calls another transition, it only
 // sets S.pc; Machine::run() dispatches until the group needs a unit
@@ -1033,34 +1046,50 @@ struct Machine {
 };
 
 // --------------------------------------------------------------------- kernel
-// STAGED: the launch has one request; its descriptor and its monomial
-// program are copied into LDS once per workgroup and every group reads
-// them from there. Otherwise requests and programs are read from global
-// memory (L2).
-// SPLIT: one path per wavefront (R = 64/G identical replicas of the group
-// that split the terms of each equation in the evaluation); otherwise one
-// path per G-lane group.
+// Lane mappings (MODE):
+//   MODE_ROW    one path per G-lane group, row per lane;
+//   MODE_SPLIT  one path per wavefront: 64/G identical replicas of the group
+//               split the terms of each equation in the evaluation;
+//   MODE_GRID   one path per G*G lanes (phc_wave.h), entry per lane.
+// STAGED: the launch has one request; its descriptor and its term program
+// are copied into LDS once per workgroup and every path reads them from
+// there. Otherwise requests and programs are read from global memory (L2).
+constexpr int MODE_ROW = 0, MODE_SPLIT = 1, MODE_GRID = 2;
+
 constexpr size_t staged_req_bytes() { return (sizeof(Req) + 15) & ~size_t(15); }
 
-template <int G, bool SPLIT>
-PHC_D size_t unit_lds_bytes(int expmax) {
-  return SPLIT ? GroupLds<G>::bytes(expmax) + SplitLds<G>::bytes()
-               : GroupLds<G>::bytes(expmax);
+template <int G, int MODE>
+constexpr int lanes_per_path() {
+  return MODE == MODE_ROW ? G : (MODE == MODE_SPLIT ? 64 : G * G);
 }
 
-template <int G, bool STAGED, bool SPLIT>
+template <int G, int MODE>
+PHC_D size_t unit_lds_bytes(int expmax, int vmax) {
+  return MODE == MODE_GRID
+             ? WaveLds<G>::bytes(expmax, vmax)
+             : (MODE == MODE_SPLIT
+                    ? GroupLds<G>::bytes(expmax) + SplitLds<G>::bytes()
+                    : GroupLds<G>::bytes(expmax));
+}
+
+template <int G, bool STAGED, int MODE>
 __global__ void __launch_bounds__(WG)
     track_kernel(Launch L) {
   extern __shared__ __align__(16) char lds_raw[];
+  constexpr bool SPLIT = MODE == MODE_SPLIT;
+  constexpr bool GRID = MODE == MODE_GRID;
+  constexpr int LP = lanes_per_path<G, MODE>();
   const int lane = threadIdx.x % G;
-  const int rep = SPLIT ? (threadIdx.x % 64) / G : 0;
-  const int owner = SPLIT ? (int)(threadIdx.x / 64) : (int)(threadIdx.x / G);
-  const int per_block = SPLIT ? (int)(blockDim.x / 64) : (int)(blockDim.x / G);
+  const int rep = (int)(threadIdx.x % LP) / G;
+  const int owner = (int)(threadIdx.x / LP);
+  const int per_block = (int)(blockDim.x / LP);
   const int slot = blockIdx.x * per_block + owner;
   const bool writer = rep == 0;
 
   Req* sreq = (Req*)lds_raw;
   TermRec* sterms = (TermRec*)(lds_raw + staged_req_bytes());
+  WTerm* swterms = (WTerm*)(lds_raw + staged_req_bytes());
+  uint4* sjobs = nullptr;
   size_t base = 0;
   if (STAGED) {
     const uint4* src = (const uint4*)&L.reqs[0];
@@ -1068,17 +1097,31 @@ __global__ void __launch_bounds__(WG)
     for (int i = threadIdx.x; i < (int)(sizeof(Req) / 16); i += blockDim.x)
       dst[i] = src[i];
     const int nt = L.reqs[0].nterms;
-    const uint4* tsrc = (const uint4*)L.reqs[0].terms;
+    const uint4* tsrc = GRID ? (const uint4*)L.reqs[0].wterms
+                             : (const uint4*)L.reqs[0].terms;
     uint4* tdst = (uint4*)sterms;
     for (int i = threadIdx.x; i < nt * 2; i += blockDim.x) tdst[i] = tsrc[i];
-    __syncthreads();
     base = staged_req_bytes() + (size_t)nt * sizeof(TermRec);
+    if (GRID) {
+      sjobs = (uint4*)(lds_raw + base);
+      const int nj = L.reqs[0].njobs * LP;
+      const uint4* jsrc = L.reqs[0].jobs;
+      for (int i = threadIdx.x; i < nj; i += blockDim.x) sjobs[i] = jsrc[i];
+      base += (size_t)nj * sizeof(uint4);
+    }
+    __syncthreads();
   }
-  char* mine = lds_raw + base + (size_t)owner * unit_lds_bytes<G, SPLIT>(L.expmax);
+  char* mine = lds_raw + base +
+               (size_t)owner * unit_lds_bytes<G, MODE>(L.expmax, L.vmax);
   GroupLds<G> gl;
-  gl.bind(mine, L.expmax);
   SplitLds<G> sl;
-  if (SPLIT) sl.bind(mine + GroupLds<G>::bytes(L.expmax));
+  WaveLds<G> wl;
+  if (GRID) {
+    wl.bind(mine, L.expmax, L.vmax);
+  } else {
+    gl.bind(mine, L.expmax);
+    if (SPLIT) sl.bind(mine + GroupLds<G>::bytes(L.expmax));
+  }
 
   PathState S;
   if (L.resume) {
@@ -1093,7 +1136,7 @@ __global__ void __launch_bounds__(WG)
       if (L.max_units && units >= L.max_units) break;
       // pop the next queue index; -1 when the queue is empty
       long long idx = -1;
-      if (SPLIT ? (threadIdx.x % 64 == 0) : (lane == 0)) {
+      if (threadIdx.x % LP == 0) {
         const unsigned long long end =
             L.items ? __atomic_load_n(L.tail, __ATOMIC_RELAXED)
                     : (unsigned long long)L.total;
@@ -1107,7 +1150,7 @@ __global__ void __launch_bounds__(WG)
           h = old;
         }
       }
-      idx = SPLIT ? __shfl(idx, 0, 64) : __shfl(idx, 0, G);
+      idx = __shfl(idx, 0, LP);
       if (idx < 0) break;
       int lo = 0;
       if (L.items) {
@@ -1148,9 +1191,21 @@ __global__ void __launch_bounds__(WG)
       cx x;
       int ok;
       double be = 0.0;
-      unit<G, SPLIT>(R, STAGED ? sterms : R.terms, lane, rep, S.pc,
-                     (lane < R.N) ? S.zq : mk(0.0, 0.0), t, dt, gl, sl, x, ok,
-                     be);
+      const cx zl = (lane < R.N) ? S.zq : mk(0.0, 0.0);
+      if (GRID) {
+        // short products (every term of degree <= 4) or any
+        if ((R.gfold & 255) <= 4)
+          unit_wave<G, 4>(R, STAGED ? swterms : R.wterms,
+                          STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
+                          dt, wl, x, ok, be);
+        else
+          unit_wave<G, 8>(R, STAGED ? swterms : R.wterms,
+                          STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
+                          dt, wl, x, ok, be);
+      }
+      else
+        unit<G, SPLIT>(R, STAGED ? sterms : R.terms, lane, rep, S.pc, zl, t,
+                       dt, gl, sl, x, ok, be);
       units++;
       if (S.pc == PC_TAN)
         M.on_tangent(x, ok);

@@ -78,24 +78,37 @@ PHC_HD_COLD cx cdiv_special(double a, double b, double c, double d,
   return mk(x, y);
 }
 
-// compiler-rt __divdc3.
-PHC_HD cx cdiv(cx num, cx den) {
-  const double a = num.re, b = num.im;
-  double c = den.re, d = den.im;
-  int ilogbw = 0;
-  const double logbw = logb(fmax(fabs(c), fabs(d)));
-  if (isfinite(logbw)) {
-    ilogbw = (int)logbw;
-    c = ldexp(c, -ilogbw);
-    d = ldexp(d, -ilogbw);
+// compiler-rt __divdc3, in two parts: the divisor's scaling (logb, the
+// scaled c and d, c^2 + d^2), which depends only on the divisor, and the
+// quotient for a numerator. cdiv(a, b) == cdiv_apply(a, cdiv_prep(b)) bit
+// for bit; a pivot used for several quotients is prepared once.
+struct cdiv_den {
+  double c, d, denom, logbw;
+  int ilogbw;
+};
+PHC_HD cdiv_den cdiv_prep(cx den) {
+  cdiv_den p;
+  p.c = den.re;
+  p.d = den.im;
+  p.ilogbw = 0;
+  p.logbw = logb(fmax(fabs(p.c), fabs(p.d)));
+  if (isfinite(p.logbw)) {
+    p.ilogbw = (int)p.logbw;
+    p.c = ldexp(p.c, -p.ilogbw);
+    p.d = ldexp(p.d, -p.ilogbw);
   }
-  const double denom = c * c + d * d;
-  const double x = ldexp((a * c + b * d) / denom, -ilogbw);
-  const double y = ldexp((b * c - a * d) / denom, -ilogbw);
+  p.denom = p.c * p.c + p.d * p.d;
+  return p;
+}
+PHC_HD cx cdiv_apply(cx num, const cdiv_den& p) {
+  const double a = num.re, b = num.im;
+  const double x = ldexp((a * p.c + b * p.d) / p.denom, -p.ilogbw);
+  const double y = ldexp((b * p.c - a * p.d) / p.denom, -p.ilogbw);
   if (isnan(x) && isnan(y))
-    return cdiv_special(a, b, c, d, denom, logbw, x, y);
+    return cdiv_special(a, b, p.c, p.d, p.denom, p.logbw, x, y);
   return mk(x, y);
 }
+PHC_HD cx cdiv(cx num, cx den) { return cdiv_apply(num, cdiv_prep(den)); }
 
 // Correction step of Borges' algorithm for 0 < ay <= ax, no overflow.
 PHC_HD double hypot_kernel(double ax, double ay) {

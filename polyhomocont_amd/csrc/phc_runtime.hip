@@ -31,7 +31,10 @@ struct phc_ctx {
   int num_cus = 0;
   int lanes_override[PHC_MAX_UNKNOWNS + 1] = {0};
   int wg_threads = 0;  // 0: choose per launch
-  int mapping = 0;     // 0: by rule, 1: one path per group, 2: per wavefront
+  // 0: by rule, 1: one path per G-lane group, 2: one path per wavefront
+  // with split terms, 3: one path per G*G-lane grid
+  int mapping = 0;
+  size_t lds_limit = 64 * 1024;  // largest LDS of one workgroup
   std::map<int, cx*> trig;  // nsamples -> device table
   unsigned long long counters[4] = {0, 0, 0, 0};  // of the last run
   std::mutex mu;
@@ -79,6 +82,8 @@ struct Arena {
 struct HostReq {
   Req r;                         // device pointers filled at upload
   std::vector<TermRec> terms;
+  std::vector<WTerm> wterms;     // grid unit tables (build_grid)
+  std::vector<uint4> jobs;
   const double* start = nullptr; // host [npaths][N][2]
   phc_endpoints* out = nullptr;
   int64_t npaths = 0;
@@ -183,6 +188,73 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
   return PHC_OK;
 }
 
+// Tables of the grid unit (phc_wave.h) for G*G lanes per path: compact
+// terms with the LDS offsets e G + j of their nonzero factors, and per lane
+// (i, j) the job records of the terms of equation i with a nonzero exponent
+// of z_j, in term order, laid out [job][lane] with lane q = j G + i. False
+// when the system does not fit the encoding (offsets above 255, 2^20
+// terms).
+bool build_grid(HostReq& h, int G) {
+  Req& r = h.r;
+  const int N = r.N, n = r.n, LP = G * G;
+  const int nt = (int)h.terms.size();
+  if (N > G || nt >= (1 << 20)) return false;
+  std::vector<WTerm> wt(nt);
+  std::vector<std::vector<uint4>> lists(LP);
+  int maxnnz = 0, maxp = 0, maxs = 0, anybase = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int k = r.first[i]; k < r.first[i + 1]; ++k) {
+      const TermRec& t = h.terms[k];
+      WTerm w;
+      memset(&w, 0, sizeof(w));
+      w.re = t.re;
+      w.im = t.im;
+      int var[16], ex[16], off[16], nz = 0;
+      for (int j = 0; j < N; ++j) {
+        const int e = (int)((t.e[j >> 2] >> (8 * (j & 3))) & 255u);
+        if (e == 0) continue;
+        const int o = e * G + j;
+        if (o > 255) return false;
+        var[nz] = j;
+        ex[nz] = e;
+        off[nz] = o;
+        w.off |= (uint64_t)o << (8 * nz);
+        ++nz;
+      }
+      w.nnz = (uint32_t)nz;
+      wt[k] = w;
+      maxnnz = std::max(maxnnz, nz);
+      for (int a = 0; a < nz; ++a) {
+        const int j = var[a], e = ex[a];
+        const int np = a, ns = nz - 1 - a;
+        uint64_t offs = e >= 2 ? (uint64_t)((e - 1) * G + j) : 0u;
+        int byte = 1;
+        for (int b = 0; b < nz; ++b)
+          if (b != a) offs |= (uint64_t)off[b] << (8 * byte++);
+        uint4 rec;
+        rec.x = (uint32_t)k | ((uint32_t)e << 20);
+        rec.y = (uint32_t)(offs & 0xffffffffu);
+        rec.z = (uint32_t)(offs >> 32);
+        rec.w = (uint32_t)np | ((uint32_t)ns << 4);
+        lists[j * G + i].push_back(rec);
+        maxp = std::max(maxp, np);
+        maxs = std::max(maxs, ns);
+        anybase |= e >= 2;
+      }
+    }
+  }
+  size_t M = 0;
+  for (const auto& l : lists) M = std::max(M, l.size());
+  h.jobs.assign(M * LP, make_uint4(WJOB_NONE, 0u, 0u, 0u));
+  for (int qq = 0; qq < LP; ++qq)
+    for (size_t m = 0; m < lists[qq].size(); ++m)
+      h.jobs[m * LP + qq] = lists[qq][m];
+  h.wterms.swap(wt);
+  r.njobs = (int32_t)M;
+  r.gfold = maxnnz | (maxp << 8) | (maxs << 16) | (anybase << 24);
+  return true;
+}
+
 // (cos, sin) of theta0_j + (m / 2^TRIG_BITS) dtheta with the host libm,
 // theta0_j = j dtheta: the angles of loop arcs (src/endgame.f90:362-363,
 // src/tracker.f90:158-159) at dyadic segment parameters.
@@ -211,32 +283,38 @@ int get_trig(phc_ctx* ctx, int nsamples, cx** out) {
   return PHC_OK;
 }
 
-template <int G, bool SPLIT>
+template <int G, int MODE>
 hipError_t launch_track(const Launch& L, bool staged, int blocks,
                         int threads, size_t lds, hipStream_t s) {
   if (staged)
-    hipLaunchKernelGGL((track_kernel<G, true, SPLIT>), dim3(blocks),
+    hipLaunchKernelGGL((track_kernel<G, true, MODE>), dim3(blocks),
                        dim3(threads), lds, s, L);
   else
-    hipLaunchKernelGGL((track_kernel<G, false, SPLIT>), dim3(blocks),
+    hipLaunchKernelGGL((track_kernel<G, false, MODE>), dim3(blocks),
                        dim3(threads), lds, s, L);
   return hipGetLastError();
 }
 
 hipError_t launch_g(int G, const Launch& L, bool staged, int blocks,
                     int threads, size_t lds, hipStream_t s,
-                    bool split = false) {
+                    int mode = MODE_ROW) {
   switch (G) {
 #ifndef PHC_ONLY_G8
     case 4:
-      return split ? launch_track<4, true>(L, staged, blocks, threads, lds, s)
-                   : launch_track<4, false>(L, staged, blocks, threads, lds, s);
+      if (mode == MODE_GRID)
+        return launch_track<4, MODE_GRID>(L, staged, blocks, threads, lds, s);
+      if (mode == MODE_SPLIT)
+        return launch_track<4, MODE_SPLIT>(L, staged, blocks, threads, lds, s);
+      return launch_track<4, MODE_ROW>(L, staged, blocks, threads, lds, s);
     case 16:
-      return launch_track<16, false>(L, staged, blocks, threads, lds, s);
+      return launch_track<16, MODE_ROW>(L, staged, blocks, threads, lds, s);
 #endif
     default:
-      return split ? launch_track<8, true>(L, staged, blocks, threads, lds, s)
-                   : launch_track<8, false>(L, staged, blocks, threads, lds, s);
+      if (mode == MODE_GRID)
+        return launch_track<8, MODE_GRID>(L, staged, blocks, threads, lds, s);
+      if (mode == MODE_SPLIT)
+        return launch_track<8, MODE_SPLIT>(L, staged, blocks, threads, lds, s);
+      return launch_track<8, MODE_ROW>(L, staged, blocks, threads, lds, s);
   }
 }
 
@@ -250,6 +328,10 @@ size_t group_lds_bytes(int G, int expmax) {
 size_t split_lds_bytes(int G) {
   const size_t R = 64 / G;
   return (R * G * (G + 1) + (size_t)G * G + G) * sizeof(cx);
+}
+size_t grid_lds_bytes(int G, int expmax, int vmax) {
+  return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)G * G + 2 * G) *
+         sizeof(cx);
 }
 
 // A prepared multi-request run for one lanes class, with device buffers.
@@ -267,7 +349,9 @@ struct ClassRun {
   int blocks = 0, threads = 0;
   size_t lds = 0;
   bool staged = false;
-  bool split = false;
+  int mode = MODE_ROW;
+  int lp = 8;     // lanes per path
+  int vmax = 0;   // grid unit: largest term count
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -293,54 +377,146 @@ void* arena_take(Arena& a, size_t bytes) {
   return p;
 }
 
+// Lane mapping, launch shape and LDS of one lanes class. Rule (mapping 0):
+// the grid unit for G <= 8 when every request fits its tables, else one
+// path per wavefront with split terms when one path per group would leave
+// most SIMDs idle, else one path per group. Mappings 1-3 force row, split
+// or grid where the class allows it.
+void plan_class(phc_ctx* ctx, ClassRun& cr) {
+  const int G = cr.G;
+  const int64_t cus = std::max(ctx->num_cus, 1);
+  cr.total = 0;
+  cr.expmax = 1;
+  cr.vmax = 0;
+  for (HostReq* h : cr.reqs) {
+    cr.total += h->npaths;
+    cr.expmax = std::max(cr.expmax, h->r.maxexp);
+    cr.vmax = std::max(cr.vmax, h->r.nterms);
+  }
+  // the grid unit also stages the right-hand side in V[lane of the path]
+  cr.vmax = std::max(cr.vmax, G * G);
+  bool grid = G <= 8 && (ctx->mapping == 0 || ctx->mapping == 3);
+  for (HostReq* h : cr.reqs)
+    if (grid) grid = build_grid(*h, G);
+  bool can_split = G <= 8;
+  for (HostReq* h : cr.reqs) can_split = can_split && h->r.N <= 64 / G;
+  const int64_t row_waves = (std::max<int64_t>(cr.total, 1) * G + 63) / 64;
+  const int64_t paths_needed = std::max<int64_t>(cr.total, 1);
+  const size_t limit = std::min<size_t>(ctx->lds_limit, 160 * 1024);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (grid)
+      cr.mode = MODE_GRID;
+    else if (can_split && (ctx->mapping == 2 ||
+                           (ctx->mapping == 0 && row_waves < 2 * 4 * cus)))
+      cr.mode = MODE_SPLIT;
+    else
+      cr.mode = MODE_ROW;
+    cr.lp = cr.mode == MODE_ROW ? G : (cr.mode == MODE_SPLIT ? 64 : G * G);
+    const size_t per_path =
+        cr.mode == MODE_GRID
+            ? grid_lds_bytes(G, cr.expmax, cr.vmax)
+            : group_lds_bytes(G, cr.expmax) +
+                  (cr.mode == MODE_SPLIT ? split_lds_bytes(G) : 0);
+    size_t prog = 0;
+    if (cr.reqs.size() == 1) {
+      prog = staged_req_bytes() + cr.reqs[0]->terms.size() * sizeof(TermRec);
+      if (cr.mode == MODE_GRID) prog += cr.reqs[0]->jobs.size() * sizeof(uint4);
+    }
+    int threads = ctx->wg_threads ? ctx->wg_threads : WG;
+    threads = std::max(threads, cr.lp);
+    if (!ctx->wg_threads) {
+      while (threads > 64 && threads > cr.lp &&
+             (paths_needed + threads / cr.lp - 1) / (threads / cr.lp) < 2 * cus)
+        threads /= 2;
+    }
+    // the grid unit keeps every path's term values in LDS: shrink the
+    // workgroup until it fits, and give up the grid mapping if one path
+    // does not
+    while (cr.mode == MODE_GRID && threads > cr.lp &&
+           (size_t)(threads / cr.lp) * per_path > limit)
+      threads /= 2;
+    if (cr.mode == MODE_GRID && (size_t)(threads / cr.lp) * per_path > limit) {
+      grid = false;
+      continue;
+    }
+    const int per_wg = threads / cr.lp;
+    int64_t blocks = (paths_needed + per_wg - 1) / per_wg;
+    blocks = std::min<int64_t>(blocks, cus * 16 * (WG / threads));
+    cr.blocks = (int)std::max<int64_t>(blocks, 1);
+    cr.threads = threads;
+    cr.lds = (size_t)per_wg * per_path;
+    const size_t stage_limit =
+        cr.mode == MODE_GRID ? limit : STAGED_LDS_LIMIT;
+    cr.staged = prog > 0 && cr.lds + prog <= stage_limit;
+    if (cr.staged) cr.lds += prog;
+    break;
+  }
+  if (cr.mode != MODE_GRID)
+    for (HostReq* h : cr.reqs) {
+      h->wterms.clear();
+      h->jobs.clear();
+      h->r.njobs = 0;
+    }
+}
+
 // Allocates device memory for all requests, uploads inputs, prepares one
 // ClassRun per lanes class.
 int stage_requests(phc_ctx* ctx, phc_staged* st) {
-  size_t need = 0;
-  const int N_any = 0;
-  (void)N_any;
   std::map<int, std::vector<HostReq*>> byG;
-  for (auto& up : st->reqs) {
-    HostReq& h = *up;
-    const int N = h.r.N;
-    need += Arena::al(h.terms.size() * sizeof(TermRec));
-    need += Arena::al((size_t)h.npaths * N * sizeof(cx)) * 2;  // start, z
-    need += Arena::al((size_t)h.npaths * 4) * 8;
-    need += Arena::al((size_t)h.npaths * 8) * 3;
-    need += 256;
-    byG[h.G].push_back(&h);
-  }
-  for (auto& kv : byG) {
-    const int G = kv.first;
-    int64_t total = 0;
-    for (auto* h : kv.second) total += h->npaths;
-    need += Arena::al(kv.second.size() * sizeof(Req));
-    need += Arena::al((kv.second.size() + 1) * sizeof(int64_t));
-    need += 256 * 3;
-    // slots for the grid (upper bound)
-    const int per_wg = WG / G;
-    const int64_t groups = std::min<int64_t>(
-        std::max<int64_t>(total, 1),
-        (int64_t)std::max(ctx->num_cus, 1) * 16 * per_wg);
-    need += Arena::al((size_t)(groups + per_wg) * G * sizeof(PathState));
-  }
-  need += 4096;
-  HIPCHK(ctx, hipMalloc(&st->arena.base, need));
-  st->arena.size = need;
-
+  for (auto& up : st->reqs) byG[up->G].push_back(up.get());
+  std::vector<ClassRun> plans;
+  size_t need = 4096;
   for (auto& kv : byG) {
     ClassRun cr;
     cr.G = kv.first;
     cr.reqs = kv.second;
+    plan_class(ctx, cr);
+    for (HostReq* h : cr.reqs) {
+      const int N = h->r.N;
+      need += Arena::al(h->terms.size() * sizeof(TermRec));
+      need += Arena::al(h->wterms.size() * sizeof(WTerm));
+      need += Arena::al(h->jobs.size() * sizeof(uint4));
+      need += Arena::al((size_t)h->npaths * N * sizeof(cx)) * 2;  // start, z
+      need += Arena::al((size_t)h->npaths * 4) * 8;
+      need += Arena::al((size_t)h->npaths * 8) * 3;
+      need += 256;
+    }
+    need += Arena::al(cr.reqs.size() * sizeof(Req));
+    need += Arena::al((cr.reqs.size() + 1) * sizeof(int64_t));
+    need += 256 * 3;
+    need += Arena::al((size_t)cr.blocks * (cr.threads / cr.lp) * cr.G *
+                      sizeof(PathState));
+    plans.push_back(cr);
+  }
+  HIPCHK(ctx, hipMalloc(&st->arena.base, need));
+  st->arena.size = need;
+
+  for (ClassRun& cr : plans) {
     std::vector<Req> host_reqs;
     std::vector<int64_t> base(1, 0);
-    for (HostReq* h : kv.second) {
+    for (HostReq* h : cr.reqs) {
       const int N = h->r.N;
       TermRec* dt = (TermRec*)arena_take(st->arena, h->terms.size() * sizeof(TermRec));
       HIPCHK(ctx, hipMemcpyAsync(dt, h->terms.data(),
                                  h->terms.size() * sizeof(TermRec),
                                  hipMemcpyHostToDevice, ctx->stream));
       h->r.terms = dt;
+      h->r.wterms = nullptr;
+      h->r.jobs = nullptr;
+      if (cr.mode == MODE_GRID) {
+        WTerm* dw = (WTerm*)arena_take(st->arena, h->wterms.size() * sizeof(WTerm));
+        HIPCHK(ctx, hipMemcpyAsync(dw, h->wterms.data(),
+                                   h->wterms.size() * sizeof(WTerm),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        h->r.wterms = dw;
+        if (!h->jobs.empty()) {
+          uint4* dj = (uint4*)arena_take(st->arena, h->jobs.size() * sizeof(uint4));
+          HIPCHK(ctx, hipMemcpyAsync(dj, h->jobs.data(),
+                                     h->jobs.size() * sizeof(uint4),
+                                     hipMemcpyHostToDevice, ctx->stream));
+          h->r.jobs = dj;
+        }
+      }
       cx* ds = (cx*)arena_take(st->arena, (size_t)h->npaths * N * sizeof(cx));
       if (h->npaths)
         HIPCHK(ctx, hipMemcpyAsync(ds, h->start,
@@ -366,9 +542,7 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
       }
       host_reqs.push_back(h->r);
       base.push_back(base.back() + h->npaths);
-      cr.expmax = std::max(cr.expmax, h->r.maxexp);
     }
-    cr.total = base.back();
     cr.d_reqs = (Req*)arena_take(st->arena, host_reqs.size() * sizeof(Req));
     HIPCHK(ctx, hipMemcpyAsync(cr.d_reqs, host_reqs.data(),
                                host_reqs.size() * sizeof(Req),
@@ -380,43 +554,9 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
     cr.d_next = (unsigned long long*)arena_take(st->arena, 8);
     cr.d_counters = (unsigned int*)arena_take(st->arena, 16);
     cr.d_active = (unsigned int*)arena_take(st->arena, 4);
-    // Lane mapping: one path per G-lane group, or -- when that would leave
-    // most SIMDs idle and every system has N <= 64/G -- one path per
-    // wavefront with the terms split over 64/G replicas.
-    const int G = cr.G;
-    const int64_t cus = std::max(ctx->num_cus, 1);
-    bool can_split = G <= 8;
-    for (HostReq* h : cr.reqs) can_split = can_split && h->r.N <= 64 / G;
-    const int64_t row_waves = (std::max<int64_t>(cr.total, 1) * G + 63) / 64;
-    cr.split = can_split && (ctx->mapping == 2 ||
-                             (ctx->mapping == 0 && row_waves < 2 * 4 * cus));
-    const int lanes_per_path = cr.split ? 64 : G;
-    int threads = ctx->wg_threads ? ctx->wg_threads : WG;
-    const int64_t paths_needed = std::max<int64_t>(cr.total, 1);
-    if (!ctx->wg_threads) {
-      threads = WG;
-      while (threads > 64 &&
-             (paths_needed + threads / lanes_per_path - 1) /
-                     (threads / lanes_per_path) < 2 * cus)
-        threads /= 2;
-    }
-    const int per_wg = threads / lanes_per_path;
-    int64_t blocks = (paths_needed + per_wg - 1) / per_wg;
-    blocks = std::min<int64_t>(blocks, cus * 16 * (WG / threads));
-    cr.blocks = (int)std::max<int64_t>(blocks, 1);
-    cr.threads = threads;
-    cr.lds = (size_t)per_wg * (group_lds_bytes(G, cr.expmax) +
-                               (cr.split ? split_lds_bytes(G) : 0));
-    if (cr.reqs.size() == 1) {
-      const size_t prog = staged_req_bytes() +
-                          cr.reqs[0]->terms.size() * sizeof(TermRec);
-      if (cr.lds + prog <= STAGED_LDS_LIMIT) {
-        cr.staged = true;
-        cr.lds += prog;
-      }
-    }
     cr.d_slots = (PathState*)arena_take(
-        st->arena, (size_t)cr.blocks * per_wg * G * sizeof(PathState));
+        st->arena,
+        (size_t)cr.blocks * (cr.threads / cr.lp) * cr.G * sizeof(PathState));
     st->classes.push_back(cr);
   }
   return PHC_OK;
@@ -435,6 +575,7 @@ int run_classes(phc_ctx* ctx, phc_staged* st) {
     L.req_base = cr.d_base;
     L.nreq = (int)cr.reqs.size();
     L.expmax = cr.expmax;
+    L.vmax = cr.vmax;
     L.next = cr.d_next;
     L.total = cr.total;
     L.slots = nullptr;
@@ -446,7 +587,7 @@ int run_classes(phc_ctx* ctx, phc_staged* st) {
     if (!cr.ev1) HIPCHK(ctx, hipEventCreate(&cr.ev1));
     HIPCHK(ctx, hipEventRecord(cr.ev0, ctx->stream));
     HIPCHK(ctx, launch_g(cr.G, L, cr.staged, cr.blocks, cr.threads, cr.lds,
-                         ctx->stream, cr.split));
+                         ctx->stream, cr.mode));
     HIPCHK(ctx, hipEventRecord(cr.ev1, ctx->stream));
   }
   return PHC_OK;
@@ -467,7 +608,7 @@ int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
     HIPCHK(ctx, hipMemcpyAsync(c, cr.d_counters, sizeof(c),
                                hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->counters[0] += c[0] / (unsigned)cr.G;  // every lane of a group adds
+    ctx->counters[0] += c[0] / (unsigned)cr.lp;  // every lane of a path adds
     ctx->counters[1] += c[1];                   // lane 0 of a group adds
   }
   for (auto& up : st->reqs) {
@@ -543,6 +684,7 @@ int phc_ctx_create(int device, phc_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) != hipSuccess)
     return PHC_ERR_DEVICE;
   ctx->num_cus = prop.multiProcessorCount;
+  ctx->lds_limit = prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
       hipSuccess)
     return PHC_ERR_DEVICE;
@@ -574,7 +716,7 @@ int phc_ctx_counters(const phc_ctx* ctx, uint64_t* out) {
 }
 
 int phc_ctx_set_mapping(phc_ctx* ctx, int mapping, int wg_threads) {
-  if (!ctx || mapping < 0 || mapping > 2) return PHC_ERR_INVALID_INPUT;
+  if (!ctx || mapping < 0 || mapping > 3) return PHC_ERR_INVALID_INPUT;
   if (wg_threads != 0 && wg_threads != 64 && wg_threads != 128 &&
       wg_threads != 256)
     return PHC_ERR_INVALID_INPUT;
@@ -648,7 +790,8 @@ int phc_stage_launch_info(phc_ctx* ctx, phc_staged* st, int32_t* info) {
   info[2] = cr.threads;
   info[3] = (int32_t)cr.lds;
   info[4] = cr.staged ? 1 : 0;
-  info[5] = cr.split ? 1 : 0;
+  info[5] = cr.mode;
+  info[6] = cr.lp;
   return PHC_OK;
 }
 

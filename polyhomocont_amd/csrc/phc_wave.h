@@ -1,0 +1,386 @@
+// phc_wave.h -- the linearization unit with G*G lanes per path (N <= G,
+// G = 4 or 8): one path per wavefront for G = 8, four per wavefront for
+// G = 4.
+//
+// Layout. The path's lanes form a G x G grid: lane q = j G + i holds entry
+// (i, j) of the N x N system -- row i, column j -- so column j is the
+// G-lane group j and row i is lane i of every group. Each group is a full
+// replica of the path state machine (lane i of every group holds component
+// i of the path's vectors), which is how the rest of the tracker sees it.
+//
+// Unit, at zq and t (src/homotopy.f90:488-511, src/sparse.f90:229-278,
+// src/linalg.f90:26-114):
+//   1. group 0 writes the power table T[e G + j] = zq_j^e to LDS;
+//   2. all lanes of the path evaluate the term values c_k prod(...) of all
+//      terms, round robin, into V[k];
+//   3. lane (i, j) sums dF_i/dz_j over its job list (the terms of
+//      equation i with a nonzero exponent of z_j, in term order), each job
+//      a product of the term's factors in the reference's association;
+//      lane (G-1, e) sums F_e = V[first_e ..] in term order;
+//   4. the lanes assemble H_z in place, H and H_t into LDS;
+//   5. LU with partial pivoting on the grid: group k finds the pivot of
+//      column k (DPP arg-max, first maximum in logical order), the pivot
+//      and multipliers are broadcast, every lane updates its own entry;
+//   6. every lane runs the two triangular solves redundantly on the
+//      factors read back from LDS, so no lane waits for another inside the
+//      serial recurrences; the divisor of each back-substitution step was
+//      prepared when its pivot was chosen.
+// Every product, sum and quotient is the one of the row-per-lane unit,
+// in the same order, so both units give identical bits.
+#pragma once
+
+namespace phc {
+
+// A term for the grid unit: coefficient, and the LDS offsets e G + j of the
+// factors zq_j^e of its nonzero exponents, ascending j, in bytes 0..7 of
+// `off` (`nnz` of them; unused bytes are 0, the offset of pw(0, 0) = 1).
+struct WTerm {
+  double re, im;
+  uint64_t off;
+  uint32_t nnz, pad;
+};
+static_assert(sizeof(WTerm) == 32, "staged copies move terms in 16-byte units");
+
+// Job record of lane (i, j): one term of equation i with a nonzero exponent
+// e of z_j. x = term index | e << 20; (y, z) = LDS offsets: byte 0 the
+// factor pw(e-1, j) (0 when e = 1), bytes 1.. the factors before z_j then
+// the factors after z_j, ascending; w = number before | number after << 4.
+// x = WJOB_NONE: no job.
+constexpr uint32_t WJOB_NONE = 0xFFFFFFFFu;
+
+template <int G>
+struct WaveLds {
+  cx* T;   // [(expmax + 1) G] power table, T[e G + j] = zq_j^e, T[j] = 1
+  cx* V;   // [vmax] term values
+  cx* A;   // [G G] factors, A[i G + j]
+  cx* H;   // [G] H_i
+  cx* HT;  // [G] H_t,i
+  PHC_D static size_t bytes(int expmax, int vmax) {
+    return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)G * G + 2 * G) *
+           sizeof(cx);
+  }
+  PHC_D void bind(char* base, int expmax, int vmax) {
+    T = (cx*)base;
+    V = T + (size_t)(expmax + 1) * G;
+    A = V + vmax;
+    H = A + (size_t)G * G;
+    HT = H + G;
+  }
+};
+
+// Broadcast from lane `src` of the path's LP lanes. With one path per
+// wavefront src is wavefront-uniform and the value goes through a scalar
+// register; otherwise through ds_bpermute.
+template <int LP>
+PHC_D int pbcast_i(int v, int src) {
+  if (LP == 64) return __builtin_amdgcn_readlane(v, src);
+  return __shfl(v, src, LP);
+}
+template <int LP>
+PHC_D double pbcast_d(double v, int src) {
+  if (LP == 64) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+  return __shfl(v, src, LP);
+}
+template <int LP>
+PHC_D cx pbcast_c(cx v, int src) {
+  return mk(pbcast_d<LP>(v.re, src), pbcast_d<LP>(v.im, src));
+}
+// Lane-varying source within the path.
+template <int LP>
+PHC_D cx pshfl_c(cx v, int src) {
+  return mk(__shfl(v.re, src, LP), __shfl(v.im, src, LP));
+}
+
+PHC_D int woff(uint64_t off, int a) { return (int)((off >> (8 * a)) & 255u); }
+
+// Factors that are exactly one: the reference multiplies by pw(0, j) = 1
+// for every zero exponent (src/sparse.f90:262, :266); this unit multiplies
+// by 1 only to pad a lane's product to the length shared by the
+// wavefront, and skips it elsewhere. Either way the result is the same up
+// to the sign of a zero component.
+// FM bounds the factors of one product (a term's nonzero exponents, a
+// job's prefix or suffix); the gathers below are unrolled to FM so that all
+// of a product's loads are in flight together.
+template <int G, int FM>
+PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
+                     const uint4* __restrict__ jobs, const int lane,
+                     const int rep, const int mode, const cx zq, const cx t,
+                     const cx dt, const WaveLds<G>& W, cx& xout, int& okout,
+                     double& be) {
+  constexpr int LP = G * G;
+  const int N = R.N, n = R.n;
+  const int q = rep * G + lane;
+  cx* __restrict__ T = W.T;
+  cx* __restrict__ V = W.V;
+  const int maxnnz = R.gfold & 255;
+  const int maxp = (R.gfold >> 8) & 255;
+  const int maxs = (R.gfold >> 16) & 255;
+  const bool anybase = (R.gfold >> 24) & 1;
+
+  // 1. power table (src/sparse.f90:251-255); row 0 holds pw(0, j) = 1
+  wave_lds_fence();
+  if (rep == 0) {
+    T[lane] = mk(1.0, 0.0);
+    if (lane < N) {
+      cx p = zq;
+      T[G + lane] = p;
+      for (int e = 2; e <= R.maxexp; ++e) {
+        p = cmul(p, zq);
+        T[e * G + lane] = p;
+      }
+    }
+  }
+  wave_lds_fence();
+
+  // 2. term values c_k pre(m) (src/sparse.f90:260-264, :270)
+  {
+    const int nf = maxnnz > 0 ? maxnnz : 1;
+    WTerm wn;
+    if (q < R.nterms) wn = wt[q];
+    for (int k = q; k < R.nterms; k += LP) {
+      const WTerm w = wn;
+      if (k + LP < R.nterms) wn = wt[k + LP];
+      cx f[FM];
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+        if (a < nf) f[a] = T[woff(w.off, a)];
+      cx P = f[0];
+#pragma unroll
+      for (int a = 1; a < FM; ++a)
+        if (a < nf) P = cmul(P, f[a]);
+      V[k] = cmul(mk(w.re, w.im), P);
+    }
+  }
+  wave_lds_fence();
+
+  // 3. Jacobian entries (src/sparse.f90:271-276) in the job lanes; F_e in
+  // lane (G-1, e) as the ordered sum of its equation's values, four per
+  // iteration.
+  const bool jl = lane < n && rep < N;       // Jacobian entry (lane, rep)
+  const bool fl = lane == G - 1 && rep < n;  // value of equation rep
+  cx acc = mk(0.0, 0.0);
+  {
+    const int M = R.njobs;
+    int fk = fl ? R.first[rep] : 0;
+    const int fk1 = fl ? R.first[rep + 1] : 0;
+    uint4 jn = make_uint4(WJOB_NONE, 0u, 0u, 0u);
+    if (M > 0) jn = jobs[q];
+    for (int m = 0; m < M; ++m) {
+      const uint4 job = jn;
+      if (m + 1 < M) jn = jobs[(size_t)(m + 1) * LP + q];
+      // F lanes: next four values
+      cx v0 = mk(0.0, 0.0), v1 = v0, v2 = v0, v3 = v0;
+      if (fk < fk1) v0 = V[fk];
+      if (fk + 1 < fk1) v1 = V[fk + 1];
+      if (fk + 2 < fk1) v2 = V[fk + 2];
+      if (fk + 3 < fk1) v3 = V[fk + 3];
+      if (job.x != WJOB_NONE) {
+        const int e = (int)(job.x >> 20);
+        const uint64_t off = ((uint64_t)job.z << 32) | job.y;
+        const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
+        // gather: coefficient, pw(e-1, j), prefix and suffix factors
+        // (suffix in right-fold order), padded with pw(0, 0) = 1
+        const WTerm& w = wt[job.x & 0xFFFFFu];
+        const cx c = mk(w.re, w.im);
+        cx B = mk(1.0, 0.0);
+        if (anybase) B = T[woff(off, 0)];
+        cx Pf[FM], Sf[FM];
+#pragma unroll
+        for (int b = 0; b < FM; ++b) {
+          if (b < maxp) Pf[b] = T[b < np ? woff(off, 1 + b) : 0];
+          if (b < maxs) Sf[b] = T[b < ns ? woff(off, np + ns - b) : 0];
+        }
+        cx d = cscale((double)e, c);
+        if (anybase) d = cmul(d, B);
+        if (maxp > 0) {
+          cx P = Pf[0];
+#pragma unroll
+          for (int b = 1; b < FM; ++b)
+            if (b < maxp) P = cmul(P, Pf[b]);
+          d = cmul(d, P);
+        }
+        if (maxs > 0) {
+          cx S = Sf[0];
+#pragma unroll
+          for (int b = 1; b < FM; ++b)
+            if (b < maxs) S = cmul(Sf[b], S);
+          d = cmul(d, S);
+        }
+        acc = cadd(acc, d);
+      }
+      if (fk < fk1) acc = cadd(acc, v0);
+      if (fk + 1 < fk1) acc = cadd(acc, v1);
+      if (fk + 2 < fk1) acc = cadd(acc, v2);
+      if (fk + 3 < fk1) acc = cadd(acc, v3);
+      fk += 4;
+    }
+    for (; fk < fk1; ++fk) acc = cadd(acc, V[fk]);
+  }
+
+  // 4. homotopy (src/homotopy.f90:491-511) and the patch row; the lane
+  // that computes H_i and H_t,i keeps the right-hand side of row i in rhs:
+  // tangent -(H_t dt/ds), Newton -H (src/tracker.f90:314, :348)
+  cx a = mk(0.0, 0.0);
+  cx rhs = mk(0.0, 0.0);
+  {
+    const cx gt = cmul(R.gamma, t);
+    const cx omt = mk(1.0 - t.re, 0.0 - t.im);
+    if (jl || fl) {
+      const int i = jl ? lane : rep;
+      const int d = R.deg[i];
+      const cx zi = T[G + i];
+      const cx w = T[G + n];
+      const cx zd1 = zpowi(zi, d - 1);
+      const cx wd1 = zpowi(w, d - 1);
+      const cx bi = R.b[i];
+      if (jl) {
+        a = cmul(omt, acc);
+        if (rep == lane) a = cadd(a, cmul(cscale((double)d, gt), zd1));
+        if (rep == n) a = csub(a, cmul(cscale((double)d, cmul(gt, bi)), wd1));
+      } else {
+        const cx g = csub(cmul(zd1, zi), cmul(cmul(bi, wd1), w));
+        const cx h = cadd(cmul(gt, g), cmul(omt, acc));
+        const cx ht = csub(cmul(R.gamma, g), acc);
+        W.H[i] = h;
+        W.HT[i] = ht;
+        rhs = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
+      }
+    }
+    if (lane == n && rep < N) a = R.patch[rep];
+    if (lane == n && rep == n) {
+      cx ssum = mk(0.0, 0.0);
+      for (int j = 0; j < N; ++j) ssum = cadd(ssum, cmul(R.patch[j], T[G + j]));
+      const cx h = mk(ssum.re - 1.0, ssum.im - 0.0);
+      const cx ht = mk(0.0, 0.0);
+      W.H[n] = h;
+      W.HT[n] = ht;
+      rhs = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
+    }
+  }
+  const bool in = lane < N && rep < N;
+  if (mode == PC_BE) {
+    if (in) W.A[lane * G + rep] = a;
+    wave_lds_fence();
+    // ||H(z,0)|| / max(max_i sum_j |H_z(i,j)| * ||z||, tiny)
+    // (src/endgame.f90:269-285); every group computes it
+    double hv = 0.0, rs = 0.0;
+    if (lane < N) {
+      hv = cabs_(W.H[lane]);
+      for (int j = 0; j < N; ++j) rs = rs + cabs_(W.A[lane * G + j]);
+    }
+    const double hn = gmax<G>(hv);
+    const double rmax = gmax<G>(rs);
+    const double zn = gnorm<G>(zq, lane, N);
+    be = hn / fmax(rmax * zn, DBL_MIN);
+    okout = 1;
+    xout = mk(0.0, 0.0);
+    return;
+  }
+
+  // 5. LU (src/linalg.f90:49-76) with logical row swaps: pos is the
+  // logical position of this lane's row; pivot ties go to the smallest
+  // position, as in the reference's first-maximum scan.
+  int pos = lane < N ? lane : (1 << 20);
+  int PL[G];
+  cdiv_den piv_den[G];
+  int ok = 1;
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    if (k >= N) break;
+    double bv = (in && rep == k && pos >= k) ? cabs_(a) : -1.0;
+    int bp = pos, bl = lane;
+    gargmax<G>(bv, bp, bl);
+    bv = pbcast_d<LP>(bv, k * G);
+    bp = pbcast_i<LP>(bp, k * G);
+    bl = pbcast_i<LP>(bl, k * G);
+    if (bv == 0.0) {
+      ok = 0;
+      break;
+    }
+    PL[k] = bl;
+    const cx piv = pbcast_c<LP>(a, k * G + bl);
+    int np = pos;
+    if (pos == k) np = bp;
+    if (lane == bl) np = k;
+    pos = np;
+    piv_den[k] = cdiv_prep(piv);
+    const cx l = cdiv_apply(a, piv_den[k]);
+    if (in && rep == k && pos > k) a = l;
+    const cx li = pshfl_c<LP>(a, k * G + lane);
+    const cx pr = pshfl_c<LP>(a, rep * G + bl);
+    const cx upd = csub(a, cmul(li, pr));
+    if (in && rep > k && pos > k) a = upd;
+  }
+  okout = ok;
+  if (!ok) {
+    xout = mk(0.0, 0.0);
+    return;
+  }
+  // factor (i, j) and right-hand side b_i of the solve: from the lane's
+  // registers when the path is the whole wavefront, else from LDS
+  if (LP != 64) {
+    if (in) W.A[lane * G + rep] = a;
+    W.V[q] = rhs;  // the term values are no longer needed
+    wave_lds_fence();
+  }
+  auto fac = [&](int i, int j) -> cx {
+    if (LP == 64) return pbcast_c<LP>(a, j * G + i);
+    return W.A[i * G + j];
+  };
+  cx bperm[G];  // b in logical row order
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    if (k < N) {
+      const int r = PL[k];
+      const int src = r < n ? r * G + G - 1 : n * G + n;
+      bperm[k] = (LP == 64) ? pbcast_c<LP>(rhs, src) : W.V[src];
+    }
+  }
+
+  // 6. permuted right-hand side, forward with unit L, back with U
+  // (src/linalg.f90:93-111), sums in the reference's order, every lane the
+  // whole recurrence.
+  cx y[G], s[G];
+  bool sset[G];
+#pragma unroll
+  for (int p = 0; p < G; ++p) sset[p] = false;
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    if (k >= N) break;
+    const cx b = bperm[k];
+    y[k] = (k == 0 || !sset[k]) ? b : csub(b, s[k]);
+#pragma unroll
+    for (int p = k + 1; p < G; ++p) {
+      if (p >= N) break;
+      const cx pr = cmul(fac(PL[p], k), y[k]);
+      s[p] = sset[p] ? cadd(s[p], pr) : pr;
+      sset[p] = true;
+    }
+  }
+  cx x[G];
+  cx xmine = mk(0.0, 0.0);
+#pragma unroll
+  for (int i = G - 1; i >= 0; --i) {
+    if (i < N) {
+      const int r = PL[i];
+      cx num = y[i];
+      if (i + 1 < N) {
+        cx sm = cmul(fac(r, i + 1), x[i + 1]);
+#pragma unroll
+        for (int j = i + 2; j < G; ++j)
+          if (j < N) sm = cadd(sm, cmul(fac(r, j), x[j]));
+        num = csub(y[i], sm);
+      }
+      x[i] = cdiv_apply(num, piv_den[i]);
+      if (lane == i) xmine = x[i];
+    }
+  }
+  xout = xmine;
+}
+
+}  // namespace phc

@@ -92,8 +92,8 @@ class Context:
         return dict(trig_fallbacks=int(c[0]), units=int(c[1]))
 
     def set_mapping(self, mapping=0, wg_threads=0):
-        """0: chosen per launch, 1: path per lane group, 2: path per
-        wavefront (split evaluation)."""
+        """0: chosen per launch, 1: path per G-lane group, 2: path per
+        wavefront (split evaluation), 3: path per G*G-lane grid."""
         self._check(self.L.phc_ctx_set_mapping(self.h, mapping, wg_threads),
                     "phc_ctx_set_mapping")
 
@@ -170,9 +170,11 @@ class Staged:
         self.ctx._check(self.ctx.L.phc_stage_launch_info(self.ctx.h, self.h,
                                                          info.ctypes.data),
                         "phc_stage_launch_info")
-        return dict(lanes_per_path=int(info[0]), workgroups=int(info[1]),
+        return dict(group=int(info[0]), lanes_per_path=int(info[6]),
+                    workgroups=int(info[1]),
                     threads=int(info[2]), lds_bytes=int(info[3]),
-                    staged=bool(info[4]), split=bool(info[5]))
+                    staged=bool(info[4]),
+                    mapping=("row", "split", "grid")[int(info[5])])
 
     def close(self):
         if self.h:

@@ -128,10 +128,11 @@ def test_batch_equals_separate_calls(ctx, host):
         assert_bit_exact(o, s, len(it[3]))
 
 
-@pytest.mark.parametrize("mapping", [1, 2])
+@pytest.mark.parametrize("mapping", [1, 2, 3])
 def test_lane_mappings_bit_exact(ctx, mapping):
-    """One path per lane group and one path per wavefront (terms split
-    across replicas, ordered sums in LDS) give the reference's records."""
+    """One path per lane group, one path per wavefront (terms split across
+    replicas, ordered sums in LDS) and one path per G*G-lane grid (entry per
+    lane) give the reference's records."""
     G = golden()
     from polyhomocont_amd import host
     ctx.set_mapping(mapping)
@@ -152,17 +153,32 @@ def test_lane_mappings_bit_exact(ctx, mapping):
         ctx.set_mapping(0)
 
 
-def test_c2_split_mapping_matches_reference(ctx, ref):
+@pytest.mark.parametrize("mapping", [1, 2])
+def test_c2_other_mappings_match_reference(ctx, ref, mapping):
     usys = W.config_system("c2")
     opts = abi.default_options()
     g, b, a, ns = ref.homotopy(usys, opts.seed)
     start = ref.start_points(usys, opts.seed, 0, 96)
-    ctx.set_mapping(2)
+    ctx.set_mapping(mapping)
     try:
         got = run_device(ctx, usys, opts, (g, b, a), start)
     finally:
         ctx.set_mapping(0)
     assert_bit_exact(got, ref.endpoints(usys, opts, 0, 96), 96)
+
+
+def test_grid_mapping_is_default_for_small_systems(ctx, host):
+    """N <= 8 runs on the grid unit unless another mapping is forced."""
+    from polyhomocont_amd import device
+    for usys in (W.config_system("c1"), W.ref_test_systems()["cubic"]):
+        opts = abi.default_options()
+        g, b, a, ns = host.homotopy(usys, opts.seed)
+        st = device.Staged(ctx, host.pack(usys), abi.HomotopyData(g, b, a),
+                           opts, host.start_points(usys, opts.seed))
+        try:
+            assert st.launch_info()["mapping"] == "grid"
+        finally:
+            st.close()
 
 
 def _wide_systems():
