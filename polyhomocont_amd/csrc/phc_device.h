@@ -62,8 +62,8 @@ struct alignas(16) Req {
   int32_t njobs;     // grid unit: job rows (0: no grid tables)
   int32_t gfold;     // grid unit: max nonzero factors of a term (bits
                      // 0-7), of a job's prefix (8-15) and suffix (16-23),
-                     // 1 << 24 if some exponent is >= 2
-  int32_t reserved1;
+                     // product length of the short job layout (24-27)
+  int32_t vstride;   // grid unit: V entries per equation
   int32_t first[PHC_MAX_UNKNOWNS + 1];
   int32_t deg[PHC_MAX_UNKNOWNS];
   cx gamma;
@@ -1096,7 +1096,8 @@ __global__ void __launch_bounds__(WG)
     uint4* dst = (uint4*)sreq;
     for (int i = threadIdx.x; i < (int)(sizeof(Req) / 16); i += blockDim.x)
       dst[i] = src[i];
-    const int nt = L.reqs[0].nterms;
+    // the grid unit's terms end with a zero-coefficient term
+    const int nt = L.reqs[0].nterms + (GRID ? 1 : 0);
     const uint4* tsrc = GRID ? (const uint4*)L.reqs[0].wterms
                              : (const uint4*)L.reqs[0].terms;
     uint4* tdst = (uint4*)sterms;
@@ -1118,6 +1119,10 @@ __global__ void __launch_bounds__(WG)
   WaveLds<G> wl;
   if (GRID) {
     wl.bind(mine, L.expmax, L.vmax);
+    // padding of the term values must read as zero
+    for (int i = (int)(threadIdx.x % LP); i < L.vmax; i += LP)
+      wl.V[i] = mk(0.0, 0.0);
+    wave_lds_fence();
   } else {
     gl.bind(mine, L.expmax);
     if (SPLIT) sl.bind(mine + GroupLds<G>::bytes(L.expmax));
@@ -1192,17 +1197,9 @@ __global__ void __launch_bounds__(WG)
       int ok;
       double be = 0.0;
       const cx zl = (lane < R.N) ? S.zq : mk(0.0, 0.0);
-      if (GRID) {
-        // short products (every term of degree <= 4) or any
-        if ((R.gfold & 255) <= 4)
-          unit_wave<G, 4>(R, STAGED ? swterms : R.wterms,
-                          STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
-                          dt, wl, x, ok, be);
-        else
-          unit_wave<G, 8>(R, STAGED ? swterms : R.wterms,
-                          STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
-                          dt, wl, x, ok, be);
-      }
+      if (GRID)
+        unit_wave<G>(R, STAGED ? swterms : R.wterms, STAGED ? sjobs : R.jobs,
+                     lane, rep, S.pc, zl, t, dt, wl, x, ok, be);
       else
         unit<G, SPLIT>(R, STAGED ? sterms : R.terms, lane, rep, S.pc, zl, t,
                        dt, gl, sl, x, ok, be);

@@ -189,24 +189,29 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
 }
 
 // Tables of the grid unit (phc_wave.h) for G*G lanes per path: compact
-// terms with the LDS offsets e G + j of their nonzero factors, and per lane
-// (i, j) the job records of the terms of equation i with a nonzero exponent
-// of z_j, in term order, laid out [job][lane] with lane q = j G + i. False
-// when the system does not fit the encoding (offsets above 255, 2^20
-// terms).
+// terms (plus a zero-coefficient term at index nterms) with the LDS offsets
+// e G + j of their nonzero factors and their V slots, and per lane (i, j)
+// the job records of the terms of equation i with a nonzero exponent of
+// z_j in term order, laid out [job][lane] with lane q = j G + i. False when
+// the system does not fit the encoding (offsets above 255, 2^20 terms).
 bool build_grid(HostReq& h, int G) {
   Req& r = h.r;
   const int N = r.N, n = r.n, LP = G * G;
   const int nt = (int)h.terms.size();
-  if (N > G || nt >= (1 << 20)) return false;
-  std::vector<WTerm> wt(nt);
-  std::vector<std::vector<uint4>> lists(LP);
-  int maxnnz = 0, maxp = 0, maxs = 0, anybase = 0;
+  if (N > G || nt + 1 >= (1 << 20)) return false;
+  struct Job {
+    int k, e, base, np, ns;
+    int others[8];  // offsets of the other nonzero factors, ascending j
+  };
+  std::vector<WTerm> wt(nt + 1);
+  memset(wt.data(), 0, wt.size() * sizeof(WTerm));
+  std::vector<std::vector<Job>> lists(LP);
+  int maxnnz = 0, maxp = 0, maxs = 0, maxterms = 0;
   for (int i = 0; i < n; ++i) {
+    maxterms = std::max(maxterms, r.first[i + 1] - r.first[i]);
     for (int k = r.first[i]; k < r.first[i + 1]; ++k) {
       const TermRec& t = h.terms[k];
-      WTerm w;
-      memset(&w, 0, sizeof(w));
+      WTerm& w = wt[k];
       w.re = t.re;
       w.im = t.im;
       int var[16], ex[16], off[16], nz = 0;
@@ -222,36 +227,63 @@ bool build_grid(HostReq& h, int G) {
         ++nz;
       }
       w.nnz = (uint32_t)nz;
-      wt[k] = w;
       maxnnz = std::max(maxnnz, nz);
       for (int a = 0; a < nz; ++a) {
-        const int j = var[a], e = ex[a];
-        const int np = a, ns = nz - 1 - a;
-        uint64_t offs = e >= 2 ? (uint64_t)((e - 1) * G + j) : 0u;
-        int byte = 1;
-        for (int b = 0; b < nz; ++b)
-          if (b != a) offs |= (uint64_t)off[b] << (8 * byte++);
-        uint4 rec;
-        rec.x = (uint32_t)k | ((uint32_t)e << 20);
-        rec.y = (uint32_t)(offs & 0xffffffffu);
-        rec.z = (uint32_t)(offs >> 32);
-        rec.w = (uint32_t)np | ((uint32_t)ns << 4);
-        lists[j * G + i].push_back(rec);
-        maxp = std::max(maxp, np);
-        maxs = std::max(maxs, ns);
-        anybase |= e >= 2;
+        Job jb;
+        memset(&jb, 0, sizeof(jb));
+        jb.k = k;
+        jb.e = ex[a];
+        jb.base = ex[a] >= 2 ? (ex[a] - 1) * G + var[a] : 0;
+        jb.np = a;
+        jb.ns = nz - 1 - a;
+        int c = 0;
+        for (int b2 = 0; b2 < nz; ++b2)
+          if (b2 != a) jb.others[c++] = off[b2];
+        lists[var[a] * G + i].push_back(jb);
+        maxp = std::max(maxp, jb.np);
+        maxs = std::max(maxs, jb.ns);
       }
     }
   }
+  // short layout: every term has <= 4 nonzero exponents
+  const int fl = maxnnz <= 4 ? std::max(1, std::max(maxp, maxs)) : 0;
   size_t M = 0;
   for (const auto& l : lists) M = std::max(M, l.size());
-  h.jobs.assign(M * LP, make_uint4(WJOB_NONE, 0u, 0u, 0u));
-  for (int qq = 0; qq < LP; ++qq)
-    for (size_t m = 0; m < lists[qq].size(); ++m)
-      h.jobs[m * LP + qq] = lists[qq][m];
+  M = std::max(M, (size_t)((maxterms + 3) / 4));
+  const int vstride = (int)(4 * M);
+  for (int i = 0; i < n; ++i)
+    for (int k = r.first[i]; k < r.first[i + 1]; ++k)
+      wt[k].slot = (uint32_t)(i * vstride + (k - r.first[i]));
+  wt[nt].slot = (uint32_t)(n * vstride);  // never written: not a real term
+  const uint4 pad = fl ? make_uint4((uint32_t)nt | (1u << 20), 0u, 0u, 0u)
+                       : make_uint4(WJOB_NONE, 0u, 0u, 0u);
+  h.jobs.assign(M * LP, pad);
+  for (int qq = 0; qq < LP; ++qq) {
+    for (size_t m = 0; m < lists[qq].size(); ++m) {
+      const Job& jb = lists[qq][m];
+      uint4 rec = make_uint4((uint32_t)jb.k | ((uint32_t)jb.e << 20), 0u, 0u, 0u);
+      if (fl) {
+        rec.y = (uint32_t)jb.base;
+        for (int b2 = 0; b2 < jb.np; ++b2)
+          rec.y |= (uint32_t)jb.others[b2] << (8 * (1 + b2));
+        // suffix in right-fold order: last factor first
+        for (int b2 = 0; b2 < jb.ns; ++b2)
+          rec.z |= (uint32_t)jb.others[jb.np + jb.ns - 1 - b2] << (8 * b2);
+      } else {
+        uint64_t offs = (uint64_t)jb.base;
+        for (int b2 = 0; b2 < jb.np + jb.ns; ++b2)
+          offs |= (uint64_t)jb.others[b2] << (8 * (1 + b2));
+        rec.y = (uint32_t)(offs & 0xffffffffu);
+        rec.z = (uint32_t)(offs >> 32);
+        rec.w = (uint32_t)jb.np | ((uint32_t)jb.ns << 4);
+      }
+      h.jobs[m * LP + qq] = rec;
+    }
+  }
   h.wterms.swap(wt);
   r.njobs = (int32_t)M;
-  r.gfold = maxnnz | (maxp << 8) | (maxs << 16) | (anybase << 24);
+  r.vstride = vstride;
+  r.gfold = maxnnz | (maxp << 8) | (maxs << 16) | (fl << 24);
   return true;
 }
 
@@ -330,7 +362,8 @@ size_t split_lds_bytes(int G) {
   return (R * G * (G + 1) + (size_t)G * G + G) * sizeof(cx);
 }
 size_t grid_lds_bytes(int G, int expmax, int vmax) {
-  return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)G * G + 2 * G) *
+  return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G +
+          2 * G) *
          sizeof(cx);
 }
 
@@ -391,13 +424,13 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
   for (HostReq* h : cr.reqs) {
     cr.total += h->npaths;
     cr.expmax = std::max(cr.expmax, h->r.maxexp);
-    cr.vmax = std::max(cr.vmax, h->r.nterms);
   }
-  // the grid unit also stages the right-hand side in V[lane of the path]
-  cr.vmax = std::max(cr.vmax, G * G);
   bool grid = G <= 8 && (ctx->mapping == 0 || ctx->mapping == 3);
   for (HostReq* h : cr.reqs)
     if (grid) grid = build_grid(*h, G);
+  if (grid)
+    for (HostReq* h : cr.reqs)
+      cr.vmax = std::max(cr.vmax, (h->r.n + 1) * h->r.vstride);
   bool can_split = G <= 8;
   for (HostReq* h : cr.reqs) can_split = can_split && h->r.N <= 64 / G;
   const int64_t row_waves = (std::max<int64_t>(cr.total, 1) * G + 63) / 64;
@@ -420,7 +453,8 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
     size_t prog = 0;
     if (cr.reqs.size() == 1) {
       prog = staged_req_bytes() + cr.reqs[0]->terms.size() * sizeof(TermRec);
-      if (cr.mode == MODE_GRID) prog += cr.reqs[0]->jobs.size() * sizeof(uint4);
+      if (cr.mode == MODE_GRID)
+        prog += sizeof(WTerm) + cr.reqs[0]->jobs.size() * sizeof(uint4);
     }
     int threads = ctx->wg_threads ? ctx->wg_threads : WG;
     threads = std::max(threads, cr.lp);

@@ -31,32 +31,51 @@
 
 namespace phc {
 
-// A term for the grid unit: coefficient, and the LDS offsets e G + j of the
+// A term for the grid unit: coefficient; the LDS offsets e G + j of the
 // factors zq_j^e of its nonzero exponents, ascending j, in bytes 0..7 of
-// `off` (`nnz` of them; unused bytes are 0, the offset of pw(0, 0) = 1).
+// `off` (`nnz` of them; unused bytes are 0, the offset of pw(0, 0) = 1);
+// `slot`: where its value goes in V.
 struct WTerm {
   double re, im;
   uint64_t off;
-  uint32_t nnz, pad;
+  uint32_t nnz, slot;
 };
 static_assert(sizeof(WTerm) == 32, "staged copies move terms in 16-byte units");
 
-// Job record of lane (i, j): one term of equation i with a nonzero exponent
-// e of z_j. x = term index | e << 20; (y, z) = LDS offsets: byte 0 the
-// factor pw(e-1, j) (0 when e = 1), bytes 1.. the factors before z_j then
-// the factors after z_j, ascending; w = number before | number after << 4.
-// x = WJOB_NONE: no job.
+// Job records of lane (i, j): the terms of equation i with a nonzero
+// exponent e of z_j, in term order. Two layouts (uint4):
+//  short (every term has at most 4 nonzero exponents):
+//    x = term | e << 20; y = offsets [pw(e-1, j), prefix 0..2];
+//    z = offsets [suffix 0..2, -] with the suffix in right-fold order;
+//    absent factors are 0 (pw(0, 0) = 1); rows past a lane's last job
+//    point at the zero-coefficient term `nterms` with e = 1;
+//  general: x = term | e << 20, or WJOB_NONE past the last job;
+//    (y, z) = offsets [pw(e-1, j), the other factors ascending];
+//    w = prefix count | suffix count << 4.
 constexpr uint32_t WJOB_NONE = 0xFFFFFFFFu;
 
+// Request fields of the grid unit, packed in Req::gfold: largest number of
+// nonzero exponents of a term, of a job's prefix and suffix factors, and
+// the product length FL of the short layout (0: general layout).
+PHC_D int gf_maxnnz(int g) { return g & 255; }
+PHC_D int gf_maxp(int g) { return (g >> 8) & 255; }
+PHC_D int gf_maxs(int g) { return (g >> 16) & 255; }
+PHC_D int gf_fl(int g) { return (g >> 24) & 15; }
+
+// LDS of one path. V holds the term values of equation e at
+// V[e * vstride ...], zero-padded to vstride, then vstride zeros that the
+// Jacobian lanes add instead of values; it is cleared once per launch.
 template <int G>
 struct WaveLds {
   cx* T;   // [(expmax + 1) G] power table, T[e G + j] = zq_j^e, T[j] = 1
-  cx* V;   // [vmax] term values
+  cx* V;   // [vmax]
   cx* A;   // [G G] factors, A[i G + j]
   cx* H;   // [G] H_i
   cx* HT;  // [G] H_t,i
+  cx* RB;  // [G G] right-hand side by lane
   PHC_D static size_t bytes(int expmax, int vmax) {
-    return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)G * G + 2 * G) *
+    return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G +
+            2 * G) *
            sizeof(cx);
   }
   PHC_D void bind(char* base, int expmax, int vmax) {
@@ -65,6 +84,7 @@ struct WaveLds {
     A = V + vmax;
     H = A + (size_t)G * G;
     HT = H + G;
+    RB = HT + G;
   }
 };
 
@@ -97,16 +117,131 @@ PHC_D cx pshfl_c(cx v, int src) {
 }
 
 PHC_D int woff(uint64_t off, int a) { return (int)((off >> (8 * a)) & 255u); }
+PHC_D int byte_of(uint32_t w, int b) { return (int)((w >> (8 * b)) & 255u); }
+
+// Term values c_k pre(m) (src/sparse.f90:260-264, :270), round robin over
+// the path's lanes, products of FV factors padded with pw(0, 0) = 1.
+template <int G, int FV>
+PHC_D void term_values(const Req& R, const WTerm* __restrict__ wt,
+                       const cx* __restrict__ T, cx* __restrict__ V, int q) {
+  constexpr int LP = G * G;
+  WTerm wn;
+  if (q < R.nterms) wn = wt[q];
+  for (int k = q; k < R.nterms; k += LP) {
+    const WTerm w = wn;
+    if (k + LP < R.nterms) wn = wt[k + LP];
+    cx f[FV];
+#pragma unroll
+    for (int a = 0; a < FV; ++a) f[a] = T[woff(w.off, a)];
+    cx P = f[0];
+#pragma unroll
+    for (int a = 1; a < FV; ++a) P = cmul(P, f[a]);
+    V[w.slot] = cmul(mk(w.re, w.im), P);
+  }
+}
+
+// Jacobian entry of lane (i, j) (src/sparse.f90:271-276) and, in the value
+// lanes, F_e = sum of its equation's values in term order, four per step;
+// short job layout, FL factors per prefix and suffix, no branches.
+template <int G, int FL>
+PHC_D cx jac_short(const Req& R, const WTerm* __restrict__ wt,
+                   const uint4* __restrict__ jobs, const cx* __restrict__ T,
+                   const cx* __restrict__ V, int q, int vbase) {
+  constexpr int LP = G * G;
+  const int M = R.njobs;
+  cx acc = mk(0.0, 0.0);
+  uint4 jn = jobs[q];
+  for (int m = 0; m < M; ++m) {
+    const uint4 job = jn;
+    if (m + 1 < M) jn = jobs[(size_t)(m + 1) * LP + q];
+    const cx v0 = V[vbase + 4 * m], v1 = V[vbase + 4 * m + 1];
+    const cx v2 = V[vbase + 4 * m + 2], v3 = V[vbase + 4 * m + 3];
+    const WTerm& w = wt[job.x & 0xFFFFFu];
+    const cx c = mk(w.re, w.im);
+    const cx B = T[byte_of(job.y, 0)];
+    cx Pf[FL], Sf[FL];
+#pragma unroll
+    for (int b = 0; b < FL; ++b) {
+      Pf[b] = T[byte_of(job.y, 1 + b)];
+      Sf[b] = T[byte_of(job.z, b)];
+    }
+    cx d = cmul(cscale((double)(job.x >> 20), c), B);
+    cx P = Pf[0], S = Sf[0];
+#pragma unroll
+    for (int b = 1; b < FL; ++b) {
+      P = cmul(P, Pf[b]);
+      S = cmul(Sf[b], S);
+    }
+    d = cmul(cmul(d, P), S);
+    acc = cadd(acc, d);
+    acc = cadd(acc, v0);
+    acc = cadd(acc, v1);
+    acc = cadd(acc, v2);
+    acc = cadd(acc, v3);
+  }
+  return acc;
+}
+
+// The same with the general job layout (any number of factors up to 8).
+template <int G>
+PHC_D cx jac_general(const Req& R, const WTerm* __restrict__ wt,
+                     const uint4* __restrict__ jobs, const cx* __restrict__ T,
+                     const cx* __restrict__ V, int q, int vbase, int vcount) {
+  constexpr int LP = G * G;
+  constexpr int FM = 8;
+  const int maxp = gf_maxp(R.gfold), maxs = gf_maxs(R.gfold);
+  const int M = R.njobs;
+  cx acc = mk(0.0, 0.0);
+  int fk = 0;
+  uint4 jn = make_uint4(WJOB_NONE, 0u, 0u, 0u);
+  if (M > 0) jn = jobs[q];
+  for (int m = 0; m < M; ++m) {
+    const uint4 job = jn;
+    if (m + 1 < M) jn = jobs[(size_t)(m + 1) * LP + q];
+    if (job.x != WJOB_NONE) {
+      const int e = (int)((job.x >> 20) & 255u);
+      const uint64_t off = ((uint64_t)job.z << 32) | job.y;
+      const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
+      const WTerm& w = wt[job.x & 0xFFFFFu];
+      const cx c = mk(w.re, w.im);
+      const cx B = T[woff(off, 0)];
+      cx Pf[FM], Sf[FM];
+#pragma unroll
+      for (int b = 0; b < FM; ++b) {
+        if (b < maxp) Pf[b] = T[b < np ? woff(off, 1 + b) : 0];
+        if (b < maxs) Sf[b] = T[b < ns ? woff(off, np + ns - b) : 0];
+      }
+      cx d = cmul(cscale((double)e, c), B);
+      if (maxp > 0) {
+        cx P = Pf[0];
+#pragma unroll
+        for (int b = 1; b < FM; ++b)
+          if (b < maxp) P = cmul(P, Pf[b]);
+        d = cmul(d, P);
+      }
+      if (maxs > 0) {
+        cx S = Sf[0];
+#pragma unroll
+        for (int b = 1; b < FM; ++b)
+          if (b < maxs) S = cmul(Sf[b], S);
+        d = cmul(d, S);
+      }
+      acc = cadd(acc, d);
+    }
+    for (int r = 0; r < 4 && fk < vcount; ++r, ++fk)
+      acc = cadd(acc, V[vbase + fk]);
+  }
+  for (; fk < vcount; ++fk) acc = cadd(acc, V[vbase + fk]);
+  return acc;
+}
 
 // Factors that are exactly one: the reference multiplies by pw(0, j) = 1
 // for every zero exponent (src/sparse.f90:262, :266); this unit multiplies
 // by 1 only to pad a lane's product to the length shared by the
 // wavefront, and skips it elsewhere. Either way the result is the same up
-// to the sign of a zero component.
-// FM bounds the factors of one product (a term's nonzero exponents, a
-// job's prefix or suffix); the gathers below are unrolled to FM so that all
-// of a product's loads are in flight together.
-template <int G, int FM>
+// to the sign of a zero component. Zero padding of sums is exact: the
+// sums start at +0 and so are never -0.
+template <int G>
 PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
                      const uint4* __restrict__ jobs, const int lane,
                      const int rep, const int mode, const cx zq, const cx t,
@@ -117,10 +252,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   const int q = rep * G + lane;
   cx* __restrict__ T = W.T;
   cx* __restrict__ V = W.V;
-  const int maxnnz = R.gfold & 255;
-  const int maxp = (R.gfold >> 8) & 255;
-  const int maxs = (R.gfold >> 16) & 255;
-  const bool anybase = (R.gfold >> 24) & 1;
+  const int fl_len = gf_fl(R.gfold);
 
   // 1. power table (src/sparse.f90:251-255); row 0 holds pw(0, j) = 1
   wave_lds_fence();
@@ -137,90 +269,28 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   }
   wave_lds_fence();
 
-  // 2. term values c_k pre(m) (src/sparse.f90:260-264, :270)
-  {
-    const int nf = maxnnz > 0 ? maxnnz : 1;
-    WTerm wn;
-    if (q < R.nterms) wn = wt[q];
-    for (int k = q; k < R.nterms; k += LP) {
-      const WTerm w = wn;
-      if (k + LP < R.nterms) wn = wt[k + LP];
-      cx f[FM];
-#pragma unroll
-      for (int a = 0; a < FM; ++a)
-        if (a < nf) f[a] = T[woff(w.off, a)];
-      cx P = f[0];
-#pragma unroll
-      for (int a = 1; a < FM; ++a)
-        if (a < nf) P = cmul(P, f[a]);
-      V[k] = cmul(mk(w.re, w.im), P);
-    }
-  }
+  // 2. term values
+  if (fl_len != 0)
+    term_values<G, 4>(R, wt, T, V, q);
+  else
+    term_values<G, 8>(R, wt, T, V, q);
   wave_lds_fence();
 
-  // 3. Jacobian entries (src/sparse.f90:271-276) in the job lanes; F_e in
-  // lane (G-1, e) as the ordered sum of its equation's values, four per
-  // iteration.
+  // 3. Jacobian entries in lanes (i < n, j < N), F_e in lane (G-1, e)
   const bool jl = lane < n && rep < N;       // Jacobian entry (lane, rep)
   const bool fl = lane == G - 1 && rep < n;  // value of equation rep
-  cx acc = mk(0.0, 0.0);
-  {
-    const int M = R.njobs;
-    int fk = fl ? R.first[rep] : 0;
-    const int fk1 = fl ? R.first[rep + 1] : 0;
-    uint4 jn = make_uint4(WJOB_NONE, 0u, 0u, 0u);
-    if (M > 0) jn = jobs[q];
-    for (int m = 0; m < M; ++m) {
-      const uint4 job = jn;
-      if (m + 1 < M) jn = jobs[(size_t)(m + 1) * LP + q];
-      // F lanes: next four values
-      cx v0 = mk(0.0, 0.0), v1 = v0, v2 = v0, v3 = v0;
-      if (fk < fk1) v0 = V[fk];
-      if (fk + 1 < fk1) v1 = V[fk + 1];
-      if (fk + 2 < fk1) v2 = V[fk + 2];
-      if (fk + 3 < fk1) v3 = V[fk + 3];
-      if (job.x != WJOB_NONE) {
-        const int e = (int)(job.x >> 20);
-        const uint64_t off = ((uint64_t)job.z << 32) | job.y;
-        const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
-        // gather: coefficient, pw(e-1, j), prefix and suffix factors
-        // (suffix in right-fold order), padded with pw(0, 0) = 1
-        const WTerm& w = wt[job.x & 0xFFFFFu];
-        const cx c = mk(w.re, w.im);
-        cx B = mk(1.0, 0.0);
-        if (anybase) B = T[woff(off, 0)];
-        cx Pf[FM], Sf[FM];
-#pragma unroll
-        for (int b = 0; b < FM; ++b) {
-          if (b < maxp) Pf[b] = T[b < np ? woff(off, 1 + b) : 0];
-          if (b < maxs) Sf[b] = T[b < ns ? woff(off, np + ns - b) : 0];
-        }
-        cx d = cscale((double)e, c);
-        if (anybase) d = cmul(d, B);
-        if (maxp > 0) {
-          cx P = Pf[0];
-#pragma unroll
-          for (int b = 1; b < FM; ++b)
-            if (b < maxp) P = cmul(P, Pf[b]);
-          d = cmul(d, P);
-        }
-        if (maxs > 0) {
-          cx S = Sf[0];
-#pragma unroll
-          for (int b = 1; b < FM; ++b)
-            if (b < maxs) S = cmul(Sf[b], S);
-          d = cmul(d, S);
-        }
-        acc = cadd(acc, d);
-      }
-      if (fk < fk1) acc = cadd(acc, v0);
-      if (fk + 1 < fk1) acc = cadd(acc, v1);
-      if (fk + 2 < fk1) acc = cadd(acc, v2);
-      if (fk + 3 < fk1) acc = cadd(acc, v3);
-      fk += 4;
-    }
-    for (; fk < fk1; ++fk) acc = cadd(acc, V[fk]);
-  }
+  const int vs = R.vstride;                  // V entries per equation
+  const int vbase = (fl ? rep : n) * vs;
+  cx acc;
+  if (fl_len == 1)
+    acc = jac_short<G, 1>(R, wt, jobs, T, V, q, vbase);
+  else if (fl_len == 2)
+    acc = jac_short<G, 2>(R, wt, jobs, T, V, q, vbase);
+  else if (fl_len == 3)
+    acc = jac_short<G, 3>(R, wt, jobs, T, V, q, vbase);
+  else
+    acc = jac_general<G>(R, wt, jobs, T, V, q, vbase,
+                         fl ? R.first[rep + 1] - R.first[rep] : 0);
 
   // 4. homotopy (src/homotopy.f90:491-511) and the patch row; the lane
   // that computes H_i and H_t,i keeps the right-hand side of row i in rhs:
@@ -287,7 +357,6 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   // position, as in the reference's first-maximum scan.
   int pos = lane < N ? lane : (1 << 20);
   int PL[G];
-  cdiv_den piv_den[G];
   int ok = 1;
 #pragma unroll
   for (int k = 0; k < G; ++k) {
@@ -308,8 +377,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     if (pos == k) np = bp;
     if (lane == bl) np = k;
     pos = np;
-    piv_den[k] = cdiv_prep(piv);
-    const cx l = cdiv_apply(a, piv_den[k]);
+    const cx l = cdiv(a, piv);
     if (in && rep == k && pos > k) a = l;
     const cx li = pshfl_c<LP>(a, k * G + lane);
     const cx pr = pshfl_c<LP>(a, rep * G + bl);
@@ -325,7 +393,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   // registers when the path is the whole wavefront, else from LDS
   if (LP != 64) {
     if (in) W.A[lane * G + rep] = a;
-    W.V[q] = rhs;  // the term values are no longer needed
+    W.RB[q] = rhs;
     wave_lds_fence();
   }
   auto fac = [&](int i, int j) -> cx {
@@ -338,7 +406,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     if (k < N) {
       const int r = PL[k];
       const int src = r < n ? r * G + G - 1 : n * G + n;
-      bperm[k] = (LP == 64) ? pbcast_c<LP>(rhs, src) : W.V[src];
+      bperm[k] = (LP == 64) ? pbcast_c<LP>(rhs, src) : W.RB[src];
     }
   }
 
@@ -376,7 +444,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
           if (j < N) sm = cadd(sm, cmul(fac(r, j), x[j]));
         num = csub(y[i], sm);
       }
-      x[i] = cdiv_apply(num, piv_den[i]);
+      x[i] = cdiv(num, fac(r, i));
       if (lane == i) xmine = x[i];
     }
   }
