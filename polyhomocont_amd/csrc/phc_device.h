@@ -1119,7 +1119,8 @@ __global__ void __launch_bounds__(WG)
   WaveLds<G> wl;
   if (GRID) {
     wl.bind(mine, L.expmax, L.vmax);
-    // padding of the term values must read as zero
+    // padding of the term values must read as zero (a staged launch has
+    // one request, so one layout, and never writes the padding)
     for (int i = (int)(threadIdx.x % LP); i < L.vmax; i += LP)
       wl.V[i] = mk(0.0, 0.0);
     wave_lds_fence();
@@ -1177,6 +1178,12 @@ __global__ void __launch_bounds__(WG)
       }
       S.req = lo;
       const Req& R = STAGED ? *sreq : L.reqs[lo];
+      if (GRID && !STAGED) {
+        // the previous path of this slot may have come from a request
+        // whose values sat where this one's zero padding is
+        for (int i = (int)(threadIdx.x % LP); i < L.vmax; i += LP)
+          wl.V[i] = mk(0.0, 0.0);
+      }
       Machine<G> M{R, S, lane, fb, writer};
       M.start_path((lane < R.N) ? R.start[(size_t)S.path * R.N + lane]
                                 : mk(0.0, 0.0));
