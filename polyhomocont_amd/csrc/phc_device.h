@@ -1136,7 +1136,8 @@ __global__ void __launch_bounds__(WG)
     S.pc = PC_IDLE;
   }
   int units = 0;
-  unsigned int* fb = L.counters;
+  // trig-table fallbacks are counted once per path (lane 0 of replica 0)
+  unsigned int* fb = (writer && lane == 0) ? L.counters : nullptr;
   for (;;) {
     if (S.pc == PC_IDLE) {
       if (L.max_units && units >= L.max_units) break;

@@ -250,6 +250,7 @@ bool build_grid(HostReq& h, int G) {
   size_t M = 0;
   for (const auto& l : lists) M = std::max(M, l.size());
   M = std::max(M, (size_t)((maxterms + 3) / 4));
+  if (fl) M += M & 1;  // the short-layout loop takes two rows per step
   const int vstride = (int)(4 * M);
   for (int i = 0; i < n; ++i)
     for (int k = r.first[i]; k < r.first[i + 1]; ++k)
@@ -642,7 +643,7 @@ int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
     HIPCHK(ctx, hipMemcpyAsync(c, cr.d_counters, sizeof(c),
                                hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->counters[0] += c[0] / (unsigned)cr.lp;  // every lane of a path adds
+    ctx->counters[0] += c[0];  // one lane of a path adds
     ctx->counters[1] += c[1];                   // lane 0 of a group adds
   }
   for (auto& up : st->reqs) {

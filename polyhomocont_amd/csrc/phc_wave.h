@@ -21,10 +21,9 @@
 //   5. LU with partial pivoting on the grid: group k finds the pivot of
 //      column k (DPP arg-max, first maximum in logical order), the pivot
 //      and multipliers are broadcast, every lane updates its own entry;
-//   6. every lane runs the two triangular solves redundantly on the
-//      factors read back from LDS, so no lane waits for another inside the
-//      serial recurrences; the divisor of each back-substitution step was
-//      prepared when its pivot was chosen.
+//   6. the triangular solves with a row per lane (lane i of each group
+//      reads row i of the factors back from LDS); each step's value is
+//      broadcast from the lane of the row at that logical position.
 // Every product, sum and quotient is the one of the row-per-lane unit,
 // in the same order, so both units give identical bits.
 #pragma once
@@ -120,29 +119,65 @@ PHC_D int woff(uint64_t off, int a) { return (int)((off >> (8 * a)) & 255u); }
 PHC_D int byte_of(uint32_t w, int b) { return (int)((w >> (8 * b)) & 255u); }
 
 // Term values c_k pre(m) (src/sparse.f90:260-264, :270), round robin over
-// the path's lanes, products of FV factors padded with pw(0, 0) = 1.
+// the path's lanes, products of FV factors padded with pw(0, 0) = 1; two
+// terms per step so that their loads and products overlap.
+template <int G, int FV>
+PHC_D cx term_value(const WTerm& w, const cx* __restrict__ T) {
+  cx f[FV];
+#pragma unroll
+  for (int a = 0; a < FV; ++a) f[a] = T[woff(w.off, a)];
+  cx P = f[0];
+#pragma unroll
+  for (int a = 1; a < FV; ++a) P = cmul(P, f[a]);
+  return cmul(mk(w.re, w.im), P);
+}
 template <int G, int FV>
 PHC_D void term_values(const Req& R, const WTerm* __restrict__ wt,
                        const cx* __restrict__ T, cx* __restrict__ V, int q) {
   constexpr int LP = G * G;
-  WTerm wn;
-  if (q < R.nterms) wn = wt[q];
-  for (int k = q; k < R.nterms; k += LP) {
-    const WTerm w = wn;
-    if (k + LP < R.nterms) wn = wt[k + LP];
-    cx f[FV];
-#pragma unroll
-    for (int a = 0; a < FV; ++a) f[a] = T[woff(w.off, a)];
-    cx P = f[0];
-#pragma unroll
-    for (int a = 1; a < FV; ++a) P = cmul(P, f[a]);
-    V[w.slot] = cmul(mk(w.re, w.im), P);
+  const int nt = R.nterms;
+  int k = q;
+  for (; k + LP < nt; k += 2 * LP) {
+    const WTerm w0 = wt[k], w1 = wt[k + LP];
+    const cx v0 = term_value<G, FV>(w0, T);
+    const cx v1 = term_value<G, FV>(w1, T);
+    V[w0.slot] = v0;
+    V[w1.slot] = v1;
+  }
+  if (k < nt) {
+    const WTerm w0 = wt[k];
+    V[w0.slot] = term_value<G, FV>(w0, T);
   }
 }
 
+// One derivative c e pw(e-1, j) pre(j-1) suf(j+1) (src/sparse.f90:273) of
+// a short-layout job record, FL factors per prefix and suffix.
+template <int FL>
+PHC_D cx job_product(const uint4 job, const WTerm* __restrict__ wt,
+                     const cx* __restrict__ T) {
+  const WTerm& w = wt[job.x & 0xFFFFFu];
+  const cx c = mk(w.re, w.im);
+  const cx B = T[byte_of(job.y, 0)];
+  cx Pf[FL], Sf[FL];
+#pragma unroll
+  for (int b = 0; b < FL; ++b) {
+    Pf[b] = T[byte_of(job.y, 1 + b)];
+    Sf[b] = T[byte_of(job.z, b)];
+  }
+  cx d = cmul(cscale((double)(job.x >> 20), c), B);
+  cx P = Pf[0], S = Sf[0];
+#pragma unroll
+  for (int b = 1; b < FL; ++b) {
+    P = cmul(P, Pf[b]);
+    S = cmul(Sf[b], S);
+  }
+  return cmul(cmul(d, P), S);
+}
+
 // Jacobian entry of lane (i, j) (src/sparse.f90:271-276) and, in the value
-// lanes, F_e = sum of its equation's values in term order, four per step;
-// short job layout, FL factors per prefix and suffix, no branches.
+// lanes, F_e = sum of its equation's values in term order, four per job;
+// short job layout, no branches, two jobs per step (the job rows are padded
+// to an even count).
 template <int G, int FL>
 PHC_D cx jac_short(const Req& R, const WTerm* __restrict__ wt,
                    const uint4* __restrict__ jobs, const cx* __restrict__ T,
@@ -150,34 +185,20 @@ PHC_D cx jac_short(const Req& R, const WTerm* __restrict__ wt,
   constexpr int LP = G * G;
   const int M = R.njobs;
   cx acc = mk(0.0, 0.0);
-  uint4 jn = jobs[q];
-  for (int m = 0; m < M; ++m) {
-    const uint4 job = jn;
-    if (m + 1 < M) jn = jobs[(size_t)(m + 1) * LP + q];
-    const cx v0 = V[vbase + 4 * m], v1 = V[vbase + 4 * m + 1];
-    const cx v2 = V[vbase + 4 * m + 2], v3 = V[vbase + 4 * m + 3];
-    const WTerm& w = wt[job.x & 0xFFFFFu];
-    const cx c = mk(w.re, w.im);
-    const cx B = T[byte_of(job.y, 0)];
-    cx Pf[FL], Sf[FL];
+  for (int m = 0; m < M; m += 2) {
+    const uint4 j0 = jobs[(size_t)m * LP + q];
+    const uint4 j1 = jobs[(size_t)(m + 1) * LP + q];
+    cx v[8];
 #pragma unroll
-    for (int b = 0; b < FL; ++b) {
-      Pf[b] = T[byte_of(job.y, 1 + b)];
-      Sf[b] = T[byte_of(job.z, b)];
-    }
-    cx d = cmul(cscale((double)(job.x >> 20), c), B);
-    cx P = Pf[0], S = Sf[0];
+    for (int r = 0; r < 8; ++r) v[r] = V[vbase + 4 * m + r];
+    const cx d0 = job_product<FL>(j0, wt, T);
+    const cx d1 = job_product<FL>(j1, wt, T);
+    acc = cadd(acc, d0);
 #pragma unroll
-    for (int b = 1; b < FL; ++b) {
-      P = cmul(P, Pf[b]);
-      S = cmul(Sf[b], S);
-    }
-    d = cmul(cmul(d, P), S);
-    acc = cadd(acc, d);
-    acc = cadd(acc, v0);
-    acc = cadd(acc, v1);
-    acc = cadd(acc, v2);
-    acc = cadd(acc, v3);
+    for (int r = 0; r < 4; ++r) acc = cadd(acc, v[r]);
+    acc = cadd(acc, d1);
+#pragma unroll
+    for (int r = 4; r < 8; ++r) acc = cadd(acc, v[r]);
   }
   return acc;
 }
@@ -389,63 +410,55 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     xout = mk(0.0, 0.0);
     return;
   }
-  // factor (i, j) and right-hand side b_i of the solve: from the lane's
-  // registers when the path is the whole wavefront, else from LDS
-  if (LP != 64) {
-    if (in) W.A[lane * G + rep] = a;
-    W.RB[q] = rhs;
-    wave_lds_fence();
-  }
-  auto fac = [&](int i, int j) -> cx {
-    if (LP == 64) return pbcast_c<LP>(a, j * G + i);
-    return W.A[i * G + j];
+  // 6. forward and back substitution (src/linalg.f90:93-111), row per
+  // lane: lane i of every group holds row i of the factors and runs the
+  // recurrence of that row; the value of the row at logical position k is
+  // broadcast from its lane (lane PL[k] of the group). Sums in the
+  // reference's order, as in the row-per-lane unit.
+  if (in) W.A[lane * G + rep] = a;
+  W.RB[q] = rhs;
+  wave_lds_fence();
+  const bool row = lane < N;
+  cx arow[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j)
+    arow[j] = row ? W.A[lane * G + j] : mk(1.0, 0.0);
+  const cx bown = row ? W.RB[lane < n ? lane * G + G - 1 : n * G + n]
+                      : mk(0.0, 0.0);
+  auto rowcast = [&](cx v, int r) -> cx {
+    if (LP == 64) return pbcast_c<LP>(v, r);  // group 0, lane r
+    return gshfl_c<G>(v, r);
   };
-  cx bperm[G];  // b in logical row order
-#pragma unroll
-  for (int k = 0; k < G; ++k) {
-    if (k < N) {
-      const int r = PL[k];
-      const int src = r < n ? r * G + G - 1 : n * G + n;
-      bperm[k] = (LP == 64) ? pbcast_c<LP>(rhs, src) : W.RB[src];
-    }
-  }
-
-  // 6. permuted right-hand side, forward with unit L, back with U
-  // (src/linalg.f90:93-111), sums in the reference's order, every lane the
-  // whole recurrence.
-  cx y[G], s[G];
-  bool sset[G];
-#pragma unroll
-  for (int p = 0; p < G; ++p) sset[p] = false;
+  cx sacc = mk(0.0, 0.0);
+  bool sset = false;
 #pragma unroll
   for (int k = 0; k < G; ++k) {
     if (k >= N) break;
-    const cx b = bperm[k];
-    y[k] = (k == 0 || !sset[k]) ? b : csub(b, s[k]);
-#pragma unroll
-    for (int p = k + 1; p < G; ++p) {
-      if (p >= N) break;
-      const cx pr = cmul(fac(PL[p], k), y[k]);
-      s[p] = sset[p] ? cadd(s[p], pr) : pr;
-      sset[p] = true;
+    const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+    const cx yk = rowcast(cand, PL[k]);
+    if (row && pos > k) {
+      const cx p = cmul(arow[k], yk);
+      sacc = sset ? cadd(sacc, p) : p;
+      sset = true;
     }
   }
-  cx x[G];
+  const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+  cx xs[G];
   cx xmine = mk(0.0, 0.0);
 #pragma unroll
   for (int i = G - 1; i >= 0; --i) {
     if (i < N) {
-      const int r = PL[i];
-      cx num = y[i];
+      cx num = yfin;
       if (i + 1 < N) {
-        cx sm = cmul(fac(r, i + 1), x[i + 1]);
+        cx sm = cmul(arow[i + 1], xs[i + 1]);
 #pragma unroll
         for (int j = i + 2; j < G; ++j)
-          if (j < N) sm = cadd(sm, cmul(fac(r, j), x[j]));
-        num = csub(y[i], sm);
+          if (j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
+        num = csub(yfin, sm);
       }
-      x[i] = cdiv(num, fac(r, i));
-      if (lane == i) xmine = x[i];
+      const cx cand = cdiv(num, arow[i]);
+      xs[i] = rowcast(cand, PL[i]);
+      if (lane == i) xmine = xs[i];
     }
   }
   xout = xmine;
