@@ -251,6 +251,7 @@ def main():
                                "monomials/equation, 2187 paths, plain per-solve "
                                "GPU tracker" if args.workload == "c2" else args.workload,
                    "npaths": npaths, "lanes_per_path": info["lanes_per_path"],
+                   "mapping": info["mapping"],
                    "workgroups": info["workgroups"],
                    "threads_per_workgroup": info["threads"]},
         "roofline": {
@@ -260,7 +261,10 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
             "traffic": traffic,
-            "kernel": "phc::track_kernel<8>",
+            "kernel": "phc::track_kernel<%d, %s, %d> (%s mapping)" % (
+                info["group"], "true" if info["staged"] else "false",
+                {"row": 0, "split": 1, "grid": 2}[info["mapping"]],
+                info["mapping"]),
             "flops_per_launch": flops_launch,
             "avg_launch_ms": round(avg_s * 1e3, 4),
             "note": "FP64 vector ALU work (no MFMA); peak is the MI355X "
