@@ -1050,7 +1050,8 @@ struct Machine {
 //   MODE_ROW    one path per G-lane group, row per lane;
 //   MODE_SPLIT  one path per wavefront: 64/G identical replicas of the group
 //               split the terms of each equation in the evaluation;
-//   MODE_GRID   one path per G*G lanes (phc_wave.h), entry per lane.
+//   MODE_GRID   one path per min(G*G, 64) lanes (phc_wave.h), entries of
+//               the linear system spread over the lanes.
 // STAGED: the launch has one request; its descriptor and its term program
 // are copied into LDS once per workgroup and every path reads them from
 // there. Otherwise requests and programs are read from global memory (L2).
@@ -1060,7 +1061,8 @@ constexpr size_t staged_req_bytes() { return (sizeof(Req) + 15) & ~size_t(15); }
 
 template <int G, int MODE>
 constexpr int lanes_per_path() {
-  return MODE == MODE_ROW ? G : (MODE == MODE_SPLIT ? 64 : G * G);
+  return MODE == MODE_ROW ? G
+                          : (MODE == MODE_SPLIT ? 64 : GridShape<G>::LP);
 }
 
 template <int G, int MODE>
@@ -1105,7 +1107,7 @@ __global__ void __launch_bounds__(WG)
     base = staged_req_bytes() + (size_t)nt * sizeof(TermRec);
     if (GRID) {
       sjobs = (uint4*)(lds_raw + base);
-      const int nj = L.reqs[0].njobs * LP;
+      const int nj = L.reqs[0].njobs * LP * GridShape<G>::E;
       const uint4* jsrc = L.reqs[0].jobs;
       for (int i = threadIdx.x; i < nj; i += blockDim.x) sjobs[i] = jsrc[i];
       base += (size_t)nj * sizeof(uint4);

@@ -196,7 +196,8 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
 // the system does not fit the encoding (offsets above 255, 2^20 terms).
 bool build_grid(HostReq& h, int G) {
   Req& r = h.r;
-  const int N = r.N, n = r.n, LP = G * G;
+  const int N = r.N, n = r.n;
+  const int LP = std::min(G * G, 64), RG = LP / G, E = G / RG;
   const int nt = (int)h.terms.size();
   if (N > G || nt + 1 >= (1 << 20)) return false;
   struct Job {
@@ -205,7 +206,7 @@ bool build_grid(HostReq& h, int G) {
   };
   std::vector<WTerm> wt(nt + 1);
   memset(wt.data(), 0, wt.size() * sizeof(WTerm));
-  std::vector<std::vector<Job>> lists(LP);
+  std::vector<std::vector<Job>> lists(LP * E);  // slot q E + e
   int maxnnz = 0, maxp = 0, maxs = 0, maxterms = 0;
   for (int i = 0; i < n; ++i) {
     maxterms = std::max(maxterms, r.first[i + 1] - r.first[i]);
@@ -239,7 +240,8 @@ bool build_grid(HostReq& h, int G) {
         int c = 0;
         for (int b2 = 0; b2 < nz; ++b2)
           if (b2 != a) jb.others[c++] = off[b2];
-        lists[var[a] * G + i].push_back(jb);
+        const int j = var[a];
+        lists[((j % RG) * G + i) * E + j / RG].push_back(jb);
         maxp = std::max(maxp, jb.np);
         maxs = std::max(maxs, jb.ns);
       }
@@ -250,7 +252,7 @@ bool build_grid(HostReq& h, int G) {
   size_t M = 0;
   for (const auto& l : lists) M = std::max(M, l.size());
   M = std::max(M, (size_t)((maxterms + 3) / 4));
-  if (fl) M += M & 1;  // the short-layout loop takes two rows per step
+  if (fl && E == 1) M += M & 1;  // short layout: two rows per step
   const int vstride = (int)(4 * M);
   for (int i = 0; i < n; ++i)
     for (int k = r.first[i]; k < r.first[i + 1]; ++k)
@@ -258,8 +260,8 @@ bool build_grid(HostReq& h, int G) {
   wt[nt].slot = (uint32_t)(n * vstride);  // never written: not a real term
   const uint4 pad = fl ? make_uint4((uint32_t)nt | (1u << 20), 0u, 0u, 0u)
                        : make_uint4(WJOB_NONE, 0u, 0u, 0u);
-  h.jobs.assign(M * LP, pad);
-  for (int qq = 0; qq < LP; ++qq) {
+  h.jobs.assign(M * LP * E, pad);
+  for (int qq = 0; qq < LP * E; ++qq) {
     for (size_t m = 0; m < lists[qq].size(); ++m) {
       const Job& jb = lists[qq][m];
       uint4 rec = make_uint4((uint32_t)jb.k | ((uint32_t)jb.e << 20), 0u, 0u, 0u);
@@ -278,7 +280,7 @@ bool build_grid(HostReq& h, int G) {
         rec.z = (uint32_t)(offs >> 32);
         rec.w = (uint32_t)jb.np | ((uint32_t)jb.ns << 4);
       }
-      h.jobs[m * LP + qq] = rec;
+      h.jobs[m * LP * E + qq] = rec;
     }
   }
   h.wterms.swap(wt);
@@ -340,6 +342,8 @@ hipError_t launch_g(int G, const Launch& L, bool staged, int blocks,
         return launch_track<4, MODE_SPLIT>(L, staged, blocks, threads, lds, s);
       return launch_track<4, MODE_ROW>(L, staged, blocks, threads, lds, s);
     case 16:
+      if (mode == MODE_GRID)
+        return launch_track<16, MODE_GRID>(L, staged, blocks, threads, lds, s);
       return launch_track<16, MODE_ROW>(L, staged, blocks, threads, lds, s);
 #endif
     default:
@@ -426,7 +430,7 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
     cr.total += h->npaths;
     cr.expmax = std::max(cr.expmax, h->r.maxexp);
   }
-  bool grid = G <= 8 && (ctx->mapping == 0 || ctx->mapping == 3);
+  bool grid = ctx->mapping == 0 || ctx->mapping == 3;
   for (HostReq* h : cr.reqs)
     if (grid) grid = build_grid(*h, G);
   if (grid)
@@ -445,7 +449,8 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
       cr.mode = MODE_SPLIT;
     else
       cr.mode = MODE_ROW;
-    cr.lp = cr.mode == MODE_ROW ? G : (cr.mode == MODE_SPLIT ? 64 : G * G);
+    cr.lp = cr.mode == MODE_ROW ? G
+                                : (cr.mode == MODE_SPLIT ? 64 : std::min(G * G, 64));
     const size_t per_path =
         cr.mode == MODE_GRID
             ? grid_lds_bytes(G, cr.expmax, cr.vmax)

@@ -1,26 +1,29 @@
-// phc_wave.h -- the linearization unit with G*G lanes per path (N <= G,
-// G = 4 or 8): one path per wavefront for G = 8, four per wavefront for
-// G = 4.
+// phc_wave.h -- the linearization unit on a grid of lanes: LP = min(G*G,
+// 64) lanes per path (N <= G): four paths per wavefront for G = 4, one path
+// per wavefront for G = 8 and G = 16.
 //
-// Layout. The path's lanes form a G x G grid: lane q = j G + i holds entry
-// (i, j) of the N x N system -- row i, column j -- so column j is the
-// G-lane group j and row i is lane i of every group. Each group is a full
-// replica of the path state machine (lane i of every group holds component
-// i of the path's vectors), which is how the rest of the tracker sees it.
+// Layout. The path's lanes form R = LP / G groups of G lanes; lane
+// q = r G + i (group r, lane i) holds the entries (i, j) of the N x N
+// system for the E = G / R columns j = r + R e, e = 0..E-1 (E = 1 for
+// G <= 8, 4 for G = 16). Row i lives in lane i of every group. Each group
+// is a full replica of the path state machine (lane i of every group holds
+// component i of the path's vectors), which is how the rest of the
+// tracker sees it.
 //
 // Unit, at zq and t (src/homotopy.f90:488-511, src/sparse.f90:229-278,
 // src/linalg.f90:26-114):
 //   1. group 0 writes the power table T[e G + j] = zq_j^e to LDS;
 //   2. all lanes of the path evaluate the term values c_k prod(...) of all
 //      terms, round robin, into V[k];
-//   3. lane (i, j) sums dF_i/dz_j over its job list (the terms of
-//      equation i with a nonzero exponent of z_j, in term order), each job
-//      a product of the term's factors in the reference's association;
-//      lane (G-1, e) sums F_e = V[first_e ..] in term order;
+//   3. the slot of entry (i, j) sums dF_i/dz_j over its job list (the
+//      terms of equation i with a nonzero exponent of z_j, in term order),
+//      each job a product of the term's factors in the reference's
+//      association; the slot of (G-1, e) sums F_e from V in term order;
 //   4. the lanes assemble H_z in place, H and H_t into LDS;
-//   5. LU with partial pivoting on the grid: group k finds the pivot of
-//      column k (DPP arg-max, first maximum in logical order), the pivot
-//      and multipliers are broadcast, every lane updates its own entry;
+//   5. LU with partial pivoting on the grid: the group holding column k
+//      finds its pivot (DPP arg-max, first maximum in logical order), the
+//      pivot and multipliers are broadcast, every lane updates its own
+//      entries;
 //   6. the triangular solves with a row per lane (lane i of each group
 //      reads row i of the factors back from LDS); each step's value is
 //      broadcast from the lane of the row at that logical position.
@@ -65,13 +68,20 @@ PHC_D int gf_fl(int g) { return (g >> 24) & 15; }
 // V[e * vstride ...], zero-padded to vstride, then vstride zeros that the
 // Jacobian lanes add instead of values; it is cleared once per launch.
 template <int G>
+struct GridShape {
+  static constexpr int LP = (G * G < 64) ? G * G : 64;  // lanes per path
+  static constexpr int R = LP / G;                       // column groups
+  static constexpr int E = G / R;                        // columns per lane
+};
+
+template <int G>
 struct WaveLds {
   cx* T;   // [(expmax + 1) G] power table, T[e G + j] = zq_j^e, T[j] = 1
   cx* V;   // [vmax]
   cx* A;   // [G G] factors, A[i G + j]
   cx* H;   // [G] H_i
   cx* HT;  // [G] H_t,i
-  cx* RB;  // [G G] right-hand side by lane
+  cx* RB;  // [G G] right-hand side by slot (lane q, column e: q E + e)
   PHC_D static size_t bytes(int expmax, int vmax) {
     return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G +
             2 * G) *
@@ -121,7 +131,7 @@ PHC_D int byte_of(uint32_t w, int b) { return (int)((w >> (8 * b)) & 255u); }
 // Term values c_k pre(m) (src/sparse.f90:260-264, :270), round robin over
 // the path's lanes, products of FV factors padded with pw(0, 0) = 1; two
 // terms per step so that their loads and products overlap.
-template <int G, int FV>
+template <int FV>
 PHC_D cx term_value(const WTerm& w, const cx* __restrict__ T) {
   cx f[FV];
 #pragma unroll
@@ -131,22 +141,21 @@ PHC_D cx term_value(const WTerm& w, const cx* __restrict__ T) {
   for (int a = 1; a < FV; ++a) P = cmul(P, f[a]);
   return cmul(mk(w.re, w.im), P);
 }
-template <int G, int FV>
+template <int LP, int FV>
 PHC_D void term_values(const Req& R, const WTerm* __restrict__ wt,
                        const cx* __restrict__ T, cx* __restrict__ V, int q) {
-  constexpr int LP = G * G;
   const int nt = R.nterms;
   int k = q;
   for (; k + LP < nt; k += 2 * LP) {
     const WTerm w0 = wt[k], w1 = wt[k + LP];
-    const cx v0 = term_value<G, FV>(w0, T);
-    const cx v1 = term_value<G, FV>(w1, T);
+    const cx v0 = term_value<FV>(w0, T);
+    const cx v1 = term_value<FV>(w1, T);
     V[w0.slot] = v0;
     V[w1.slot] = v1;
   }
   if (k < nt) {
     const WTerm w0 = wt[k];
-    V[w0.slot] = term_value<G, FV>(w0, T);
+    V[w0.slot] = term_value<FV>(w0, T);
   }
 }
 
@@ -174,86 +183,97 @@ PHC_D cx job_product(const uint4 job, const WTerm* __restrict__ wt,
   return cmul(cmul(d, P), S);
 }
 
-// Jacobian entry of lane (i, j) (src/sparse.f90:271-276) and, in the value
-// lanes, F_e = sum of its equation's values in term order, four per job;
-// short job layout, no branches, two jobs per step (the job rows are padded
-// to an even count).
+// Jacobian entries of the lane's E slots (src/sparse.f90:271-276) and, in
+// the value slots, F_e = sum of its equation's values in term order, four
+// per job. Short job layout, no branches; U jobs per step (the job rows are
+// padded to a multiple of U). Job table [m][LP][E].
 template <int G, int FL>
-PHC_D cx jac_short(const Req& R, const WTerm* __restrict__ wt,
-                   const uint4* __restrict__ jobs, const cx* __restrict__ T,
-                   const cx* __restrict__ V, int q, int vbase) {
-  constexpr int LP = G * G;
+PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
+                     const uint4* __restrict__ jobs, const cx* __restrict__ T,
+                     const cx* __restrict__ V, int q, const int (&vbase)[GridShape<G>::E],
+                     cx (&acc)[GridShape<G>::E]) {
+  constexpr int LP = GridShape<G>::LP, E = GridShape<G>::E;
+  constexpr int U = E == 1 ? 2 : 1;
   const int M = R.njobs;
-  cx acc = mk(0.0, 0.0);
-  for (int m = 0; m < M; m += 2) {
-    const uint4 j0 = jobs[(size_t)m * LP + q];
-    const uint4 j1 = jobs[(size_t)(m + 1) * LP + q];
-    cx v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = V[vbase + 4 * m + r];
-    const cx d0 = job_product<FL>(j0, wt, T);
-    const cx d1 = job_product<FL>(j1, wt, T);
-    acc = cadd(acc, d0);
+  for (int e = 0; e < E; ++e) acc[e] = mk(0.0, 0.0);
+  for (int m = 0; m < M; m += U) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc = cadd(acc, v[r]);
-    acc = cadd(acc, d1);
+    for (int u = 0; u < U; ++u) {
+      uint4 job[E];
 #pragma unroll
-    for (int r = 4; r < 8; ++r) acc = cadd(acc, v[r]);
+      for (int e = 0; e < E; ++e)
+        job[e] = jobs[((size_t)(m + u) * LP + q) * E + e];
+      cx d[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) d[e] = job_product<FL>(job[e], wt, T);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int vb = vbase[e] + 4 * (m + u);
+        const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
+        acc[e] = cadd(acc[e], d[e]);
+        acc[e] = cadd(acc[e], v0);
+        acc[e] = cadd(acc[e], v1);
+        acc[e] = cadd(acc[e], v2);
+        acc[e] = cadd(acc[e], v3);
+      }
+    }
   }
-  return acc;
 }
 
 // The same with the general job layout (any number of factors up to 8).
 template <int G>
-PHC_D cx jac_general(const Req& R, const WTerm* __restrict__ wt,
-                     const uint4* __restrict__ jobs, const cx* __restrict__ T,
-                     const cx* __restrict__ V, int q, int vbase, int vcount) {
-  constexpr int LP = G * G;
+PHC_D void jac_general(const Req& R, const WTerm* __restrict__ wt,
+                       const uint4* __restrict__ jobs, const cx* __restrict__ T,
+                       const cx* __restrict__ V, int q,
+                       const int (&vbase)[GridShape<G>::E],
+                       const int (&vcount)[GridShape<G>::E],
+                       cx (&acc)[GridShape<G>::E]) {
+  constexpr int LP = GridShape<G>::LP, E = GridShape<G>::E;
   constexpr int FM = 8;
   const int maxp = gf_maxp(R.gfold), maxs = gf_maxs(R.gfold);
   const int M = R.njobs;
-  cx acc = mk(0.0, 0.0);
-  int fk = 0;
-  uint4 jn = make_uint4(WJOB_NONE, 0u, 0u, 0u);
-  if (M > 0) jn = jobs[q];
-  for (int m = 0; m < M; ++m) {
-    const uint4 job = jn;
-    if (m + 1 < M) jn = jobs[(size_t)(m + 1) * LP + q];
-    if (job.x != WJOB_NONE) {
-      const int e = (int)((job.x >> 20) & 255u);
-      const uint64_t off = ((uint64_t)job.z << 32) | job.y;
-      const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
-      const WTerm& w = wt[job.x & 0xFFFFFu];
-      const cx c = mk(w.re, w.im);
-      const cx B = T[woff(off, 0)];
-      cx Pf[FM], Sf[FM];
 #pragma unroll
-      for (int b = 0; b < FM; ++b) {
-        if (b < maxp) Pf[b] = T[b < np ? woff(off, 1 + b) : 0];
-        if (b < maxs) Sf[b] = T[b < ns ? woff(off, np + ns - b) : 0];
-      }
-      cx d = cmul(cscale((double)e, c), B);
-      if (maxp > 0) {
-        cx P = Pf[0];
+  for (int e = 0; e < E; ++e) {
+    acc[e] = mk(0.0, 0.0);
+    int fk = 0;
+    for (int m = 0; m < M; ++m) {
+      const uint4 job = jobs[((size_t)m * LP + q) * E + e];
+      if (job.x != WJOB_NONE) {
+        const int ex = (int)((job.x >> 20) & 255u);
+        const uint64_t off = ((uint64_t)job.z << 32) | job.y;
+        const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
+        const WTerm& w = wt[job.x & 0xFFFFFu];
+        const cx c = mk(w.re, w.im);
+        const cx B = T[woff(off, 0)];
+        cx Pf[FM], Sf[FM];
 #pragma unroll
-        for (int b = 1; b < FM; ++b)
-          if (b < maxp) P = cmul(P, Pf[b]);
-        d = cmul(d, P);
-      }
-      if (maxs > 0) {
-        cx S = Sf[0];
+        for (int b = 0; b < FM; ++b) {
+          if (b < maxp) Pf[b] = T[b < np ? woff(off, 1 + b) : 0];
+          if (b < maxs) Sf[b] = T[b < ns ? woff(off, np + ns - b) : 0];
+        }
+        cx d = cmul(cscale((double)ex, c), B);
+        if (maxp > 0) {
+          cx P = Pf[0];
 #pragma unroll
-        for (int b = 1; b < FM; ++b)
-          if (b < maxs) S = cmul(Sf[b], S);
-        d = cmul(d, S);
+          for (int b = 1; b < FM; ++b)
+            if (b < maxp) P = cmul(P, Pf[b]);
+          d = cmul(d, P);
+        }
+        if (maxs > 0) {
+          cx S = Sf[0];
+#pragma unroll
+          for (int b = 1; b < FM; ++b)
+            if (b < maxs) S = cmul(Sf[b], S);
+          d = cmul(d, S);
+        }
+        acc[e] = cadd(acc[e], d);
       }
-      acc = cadd(acc, d);
+      for (int r = 0; r < 4 && fk < vcount[e]; ++r, ++fk)
+        acc[e] = cadd(acc[e], V[vbase[e] + fk]);
     }
-    for (int r = 0; r < 4 && fk < vcount; ++r, ++fk)
-      acc = cadd(acc, V[vbase + fk]);
+    for (; fk < vcount[e]; ++fk) acc[e] = cadd(acc[e], V[vbase[e] + fk]);
   }
-  for (; fk < vcount; ++fk) acc = cadd(acc, V[vbase + fk]);
-  return acc;
 }
 
 // Factors that are exactly one: the reference multiplies by pw(0, j) = 1
@@ -268,12 +288,16 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
                      const int rep, const int mode, const cx zq, const cx t,
                      const cx dt, const WaveLds<G>& W, cx& xout, int& okout,
                      double& be) {
-  constexpr int LP = G * G;
+  constexpr int LP = GridShape<G>::LP, RG = GridShape<G>::R,
+                E = GridShape<G>::E;
   const int N = R.N, n = R.n;
   const int q = rep * G + lane;
   cx* __restrict__ T = W.T;
   cx* __restrict__ V = W.V;
   const int fl_len = gf_fl(R.gfold);
+  int col[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) col[e] = rep + RG * e;
 
   // 1. power table (src/sparse.f90:251-255); row 0 holds pw(0, j) = 1
   wave_lds_fence();
@@ -292,70 +316,84 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
 
   // 2. term values
   if (fl_len != 0)
-    term_values<G, 4>(R, wt, T, V, q);
+    term_values<LP, 4>(R, wt, T, V, q);
   else
-    term_values<G, 8>(R, wt, T, V, q);
+    term_values<LP, 8>(R, wt, T, V, q);
   wave_lds_fence();
 
-  // 3. Jacobian entries in lanes (i < n, j < N), F_e in lane (G-1, e)
-  const bool jl = lane < n && rep < N;       // Jacobian entry (lane, rep)
-  const bool fl = lane == G - 1 && rep < n;  // value of equation rep
-  const int vs = R.vstride;                  // V entries per equation
-  const int vbase = (fl ? rep : n) * vs;
-  cx acc;
+  // 3. Jacobian entries in slots (i < n, j < N), F_e in slot (G-1, e)
+  const int vs = R.vstride;  // V entries per equation
+  bool jl[E], fl[E];
+  int vbase[E], vcount[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    jl[e] = lane < n && col[e] < N;
+    fl[e] = lane == G - 1 && col[e] < n;
+    vbase[e] = (fl[e] ? col[e] : n) * vs;
+    vcount[e] = fl[e] ? R.first[col[e] + 1] - R.first[col[e]] : 0;
+  }
+  cx acc[E];
   if (fl_len == 1)
-    acc = jac_short<G, 1>(R, wt, jobs, T, V, q, vbase);
+    jac_short<G, 1>(R, wt, jobs, T, V, q, vbase, acc);
   else if (fl_len == 2)
-    acc = jac_short<G, 2>(R, wt, jobs, T, V, q, vbase);
+    jac_short<G, 2>(R, wt, jobs, T, V, q, vbase, acc);
   else if (fl_len == 3)
-    acc = jac_short<G, 3>(R, wt, jobs, T, V, q, vbase);
+    jac_short<G, 3>(R, wt, jobs, T, V, q, vbase, acc);
   else
-    acc = jac_general<G>(R, wt, jobs, T, V, q, vbase,
-                         fl ? R.first[rep + 1] - R.first[rep] : 0);
+    jac_general<G>(R, wt, jobs, T, V, q, vbase, vcount, acc);
 
-  // 4. homotopy (src/homotopy.f90:491-511) and the patch row; the lane
+  // 4. homotopy (src/homotopy.f90:491-511) and the patch row; the slot
   // that computes H_i and H_t,i keeps the right-hand side of row i in rhs:
   // tangent -(H_t dt/ds), Newton -H (src/tracker.f90:314, :348)
-  cx a = mk(0.0, 0.0);
-  cx rhs = mk(0.0, 0.0);
+  cx a[E], rhs[E];
+  bool in[E];
   {
     const cx gt = cmul(R.gamma, t);
     const cx omt = mk(1.0 - t.re, 0.0 - t.im);
-    if (jl || fl) {
-      const int i = jl ? lane : rep;
-      const int d = R.deg[i];
-      const cx zi = T[G + i];
-      const cx w = T[G + n];
-      const cx zd1 = zpowi(zi, d - 1);
-      const cx wd1 = zpowi(w, d - 1);
-      const cx bi = R.b[i];
-      if (jl) {
-        a = cmul(omt, acc);
-        if (rep == lane) a = cadd(a, cmul(cscale((double)d, gt), zd1));
-        if (rep == n) a = csub(a, cmul(cscale((double)d, cmul(gt, bi)), wd1));
-      } else {
-        const cx g = csub(cmul(zd1, zi), cmul(cmul(bi, wd1), w));
-        const cx h = cadd(cmul(gt, g), cmul(omt, acc));
-        const cx ht = csub(cmul(R.gamma, g), acc);
-        W.H[i] = h;
-        W.HT[i] = ht;
-        rhs = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = col[e];
+      a[e] = mk(0.0, 0.0);
+      rhs[e] = mk(0.0, 0.0);
+      if (jl[e] || fl[e]) {
+        const int i = jl[e] ? lane : j;
+        const int d = R.deg[i];
+        const cx zi = T[G + i];
+        const cx w = T[G + n];
+        const cx zd1 = zpowi(zi, d - 1);
+        const cx wd1 = zpowi(w, d - 1);
+        const cx bi = R.b[i];
+        if (jl[e]) {
+          a[e] = cmul(omt, acc[e]);
+          if (j == lane) a[e] = cadd(a[e], cmul(cscale((double)d, gt), zd1));
+          if (j == n) a[e] = csub(a[e], cmul(cscale((double)d, cmul(gt, bi)), wd1));
+        } else {
+          const cx g = csub(cmul(zd1, zi), cmul(cmul(bi, wd1), w));
+          const cx h = cadd(cmul(gt, g), cmul(omt, acc[e]));
+          const cx ht = csub(cmul(R.gamma, g), acc[e]);
+          W.H[i] = h;
+          W.HT[i] = ht;
+          rhs[e] = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
+        }
       }
-    }
-    if (lane == n && rep < N) a = R.patch[rep];
-    if (lane == n && rep == n) {
-      cx ssum = mk(0.0, 0.0);
-      for (int j = 0; j < N; ++j) ssum = cadd(ssum, cmul(R.patch[j], T[G + j]));
-      const cx h = mk(ssum.re - 1.0, ssum.im - 0.0);
-      const cx ht = mk(0.0, 0.0);
-      W.H[n] = h;
-      W.HT[n] = ht;
-      rhs = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
+      if (lane == n && j < N) a[e] = R.patch[j];
+      if (lane == n && j == n) {
+        cx ssum = mk(0.0, 0.0);
+        for (int jj = 0; jj < N; ++jj)
+          ssum = cadd(ssum, cmul(R.patch[jj], T[G + jj]));
+        const cx h = mk(ssum.re - 1.0, ssum.im - 0.0);
+        const cx ht = mk(0.0, 0.0);
+        W.H[n] = h;
+        W.HT[n] = ht;
+        rhs[e] = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
+      }
+      in[e] = lane < N && j < N;
     }
   }
-  const bool in = lane < N && rep < N;
   if (mode == PC_BE) {
-    if (in) W.A[lane * G + rep] = a;
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (in[e]) W.A[lane * G + col[e]] = a[e];
     wave_lds_fence();
     // ||H(z,0)|| / max(max_i sum_j |H_z(i,j)| * ||z||, tiny)
     // (src/endgame.f90:269-285); every group computes it
@@ -375,56 +413,70 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
 
   // 5. LU (src/linalg.f90:49-76) with logical row swaps: pos is the
   // logical position of this lane's row; pivot ties go to the smallest
-  // position, as in the reference's first-maximum scan.
+  // position, as in the reference's first-maximum scan. Column k is in
+  // group k % R, slot k / R.
   int pos = lane < N ? lane : (1 << 20);
   int PL[G];
   int ok = 1;
 #pragma unroll
   for (int k = 0; k < G; ++k) {
     if (k >= N) break;
-    double bv = (in && rep == k && pos >= k) ? cabs_(a) : -1.0;
+    const int kr = k % RG, ke = k / RG;
+    double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
     int bp = pos, bl = lane;
     gargmax<G>(bv, bp, bl);
-    bv = pbcast_d<LP>(bv, k * G);
-    bp = pbcast_i<LP>(bp, k * G);
-    bl = pbcast_i<LP>(bl, k * G);
+    bv = pbcast_d<LP>(bv, kr * G);
+    bp = pbcast_i<LP>(bp, kr * G);
+    bl = pbcast_i<LP>(bl, kr * G);
     if (bv == 0.0) {
       ok = 0;
       break;
     }
     PL[k] = bl;
-    const cx piv = pbcast_c<LP>(a, k * G + bl);
+    const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
     int np = pos;
     if (pos == k) np = bp;
     if (lane == bl) np = k;
     pos = np;
-    const cx l = cdiv(a, piv);
-    if (in && rep == k && pos > k) a = l;
-    const cx li = pshfl_c<LP>(a, k * G + lane);
-    const cx pr = pshfl_c<LP>(a, rep * G + bl);
-    const cx upd = csub(a, cmul(li, pr));
-    if (in && rep > k && pos > k) a = upd;
+    const cx l = cdiv(a[ke], piv);
+    if (in[ke] && rep == kr && pos > k) a[ke] = l;
+    const cx li = pshfl_c<LP>(a[ke], kr * G + lane);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const cx pr = pshfl_c<LP>(a[e], rep * G + bl);
+      const cx upd = csub(a[e], cmul(li, pr));
+      if (in[e] && col[e] > k && pos > k) a[e] = upd;
+    }
   }
   okout = ok;
   if (!ok) {
     xout = mk(0.0, 0.0);
     return;
   }
+
   // 6. forward and back substitution (src/linalg.f90:93-111), row per
   // lane: lane i of every group holds row i of the factors and runs the
   // recurrence of that row; the value of the row at logical position k is
   // broadcast from its lane (lane PL[k] of the group). Sums in the
   // reference's order, as in the row-per-lane unit.
-  if (in) W.A[lane * G + rep] = a;
-  W.RB[q] = rhs;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (in[e]) W.A[lane * G + col[e]] = a[e];
+    W.RB[q * E + e] = rhs[e];
+  }
   wave_lds_fence();
   const bool row = lane < N;
   cx arow[G];
 #pragma unroll
   for (int j = 0; j < G; ++j)
     arow[j] = row ? W.A[lane * G + j] : mk(1.0, 0.0);
-  const cx bown = row ? W.RB[lane < n ? lane * G + G - 1 : n * G + n]
-                      : mk(0.0, 0.0);
+  cx bown = mk(0.0, 0.0);
+  if (row) {
+    // the slot that holds row `lane`'s right-hand side
+    const int oc = lane < n ? lane : n;             // its column
+    const int orow = lane < n ? G - 1 : n;          // its row
+    bown = W.RB[((oc % RG) * G + orow) * E + oc / RG];
+  }
   auto rowcast = [&](cx v, int r) -> cx {
     if (LP == 64) return pbcast_c<LP>(v, r);  // group 0, lane r
     return gshfl_c<G>(v, r);
