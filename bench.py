@@ -174,6 +174,32 @@ def serving_benchmark(ctx, host_lib, count):
     }
 
 
+def batch_benchmark(ctx, host_lib, count):
+    """The batched multi-system launch: the first `count` of the seeded
+    random sparse systems (n 3..8, degree 2..4, 5..40 terms per equation),
+    all their paths in one persistent launch per lanes class."""
+    from polyhomocont_amd import abi, workloads
+    items = []
+    for usys in workloads.batch_systems(count):
+        opts = abi.default_options()
+        g, b, a, ns = host_lib.homotopy(usys, opts.seed)
+        items.append((host_lib.pack(usys), abi.HomotopyData(g, b, a), opts,
+                      host_lib.start_points(usys, opts.seed)))
+    paths = sum(len(it[3]) for it in items)
+    log(f"batch: {count} systems, {paths} paths")
+    t0 = time.perf_counter()
+    outs = ctx.track_batch(items)
+    dt = time.perf_counter() - t0
+    ok = sum(int((o.status[:len(it[3])] == 0).sum()) for o, it in zip(outs, items))
+    log(f"batch: {dt:.2f} s")
+    return {"workload": f"{count} random sparse systems (n 3..8, degree 2..4, "
+                        "5..40 terms per equation), one launch per lanes class",
+            "systems": count, "paths": paths,
+            "paths_per_s": round(paths / dt, 1), "seconds": round(dt, 3),
+            "paths_success": ok,
+            "includes": "host packing excluded; upload, kernels, download included"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +210,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=96,
+                    help="systems of the batched multi-system launch (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; the path "
@@ -205,6 +234,10 @@ def main():
     hom = abi.HomotopyData(gamma, b, patch)
 
     ctx = device.Context(0)
+    tune = None
+    if not args.no_autotune:
+        tune = ctx.autotune(packed, hom, opts, start)
+        log(f"autotune: {tune}")
     st = device.Staged(ctx, packed, hom, opts, start)
     log(f"staged {args.workload}: {npaths} paths, launch {st.launch_info()}")
     for _ in range(args.warmup):
@@ -252,6 +285,7 @@ def main():
                                "GPU tracker" if args.workload == "c2" else args.workload,
                    "npaths": npaths, "lanes_per_path": info["lanes_per_path"],
                    "mapping": info["mapping"],
+                   "autotuned": tune is not None,
                    "workgroups": info["workgroups"],
                    "threads_per_workgroup": info["threads"]},
         "roofline": {
@@ -289,6 +323,8 @@ def main():
                          "classification_equal": bool(same),
                          "bit_identical_fraction": bit}
         out["gpu_vs_cpu_core"] = round(value / cb["value"], 1)
+    if args.batch_systems > 0:
+        out["batch"] = batch_benchmark(ctx, H, args.batch_systems)
     if not args.no_serving:
         out["serving"] = serving_benchmark(ctx, H, args.serving_requests)
     print(json.dumps(out), flush=True)

@@ -229,6 +229,22 @@ int phc_stage_elapsed(phc_ctx *ctx, phc_staged *st, double *ms);
 int phc_stage_launch_info(phc_ctx *ctx, phc_staged *st, int32_t *info);
 int phc_ctx_sync(phc_ctx *ctx);
 
+/* Autotuner of the launch shape. Times, on the first min(npaths, 4096)
+ * start points of this request, every lane mapping the request allows (grid,
+ * row, split) with 64-, 128- and 256-thread workgroups, its program staged
+ * in LDS or read from L2, and keeps the fastest for later single-request
+ * launches of the same shape (lanes class, N, term count, log2 of the path
+ * count) while the mapping is 0 (chosen per launch). choice[0] mapping
+ * (1 row, 2 split, 3 grid), [1] threads per workgroup, [2] 1 if staged;
+ * *best_ms (may be NULL) the winner's kernel time. Results never depend on
+ * the choice. No counterpart in the reference (its tracker is serial). */
+int phc_ctx_autotune(phc_ctx *ctx, const phc_system *sys,
+                     const phc_homotopy *hom, const phc_solve_options *opts,
+                     int64_t npaths, const double *start, int32_t *choice,
+                     double *best_ms);
+/* Forgets every autotuned choice. */
+int phc_ctx_clear_tuning(phc_ctx *ctx);
+
 /* ---- batched multi-system tracker ------------------------------------- */
 /* Tracks the paths of nsys independent solves in one persistent launch
  * sequence; system s has npaths[s] paths with start points start[s] and

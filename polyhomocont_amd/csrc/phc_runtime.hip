@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <chrono>
 #include <cmath>
@@ -34,7 +35,12 @@ struct phc_ctx {
   // 0: by rule, 1: one path per G-lane group, 2: one path per wavefront
   // with split terms, 3: one path per G*G-lane grid
   int mapping = 0;
+  int staging = -1;    // program in LDS: -1 when it fits, 0 never
   size_t lds_limit = 64 * 1024;  // largest LDS of one workgroup
+  // autotuned launch choices of single-request runs, keyed by the request's
+  // shape (G, N, terms, log2 of the path count): mapping 1-3, workgroup
+  // threads, staging (phc_ctx_autotune)
+  std::map<std::array<int64_t, 4>, std::array<int, 3>> tuned;
   std::map<int, cx*> trig;  // nsamples -> device table
   unsigned long long counters[4] = {0, 0, 0, 0};  // of the last run
   std::mutex mu;
@@ -420,8 +426,24 @@ void* arena_take(Arena& a, size_t bytes) {
 // path per wavefront with split terms when one path per group would leave
 // most SIMDs idle, else one path per group. Mappings 1-3 force row, split
 // or grid where the class allows it.
+std::array<int64_t, 4> tune_key(const HostReq& h) {
+  int64_t b = 0;
+  while ((int64_t(1) << (b + 1)) <= std::max<int64_t>(h.npaths, 1)) ++b;
+  return {h.G, h.r.N, h.r.nterms, b};
+}
+
 void plan_class(phc_ctx* ctx, ClassRun& cr) {
   const int G = cr.G;
+  int mapping = ctx->mapping, wg_threads = ctx->wg_threads,
+      staging = ctx->staging;
+  if (mapping == 0 && cr.reqs.size() == 1) {
+    auto it = ctx->tuned.find(tune_key(*cr.reqs[0]));
+    if (it != ctx->tuned.end()) {
+      mapping = it->second[0];
+      wg_threads = it->second[1];
+      staging = it->second[2];
+    }
+  }
   const int64_t cus = std::max(ctx->num_cus, 1);
   cr.total = 0;
   cr.expmax = 1;
@@ -430,7 +452,7 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
     cr.total += h->npaths;
     cr.expmax = std::max(cr.expmax, h->r.maxexp);
   }
-  bool grid = ctx->mapping == 0 || ctx->mapping == 3;
+  bool grid = mapping == 0 || mapping == 3;
   for (HostReq* h : cr.reqs)
     if (grid) grid = build_grid(*h, G);
   if (grid)
@@ -444,8 +466,8 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
   for (int attempt = 0; attempt < 2; ++attempt) {
     if (grid)
       cr.mode = MODE_GRID;
-    else if (can_split && (ctx->mapping == 2 ||
-                           (ctx->mapping == 0 && row_waves < 2 * 4 * cus)))
+    else if (can_split && (mapping == 2 ||
+                           (mapping == 0 && row_waves < 2 * 4 * cus)))
       cr.mode = MODE_SPLIT;
     else
       cr.mode = MODE_ROW;
@@ -462,9 +484,9 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
       if (cr.mode == MODE_GRID)
         prog += sizeof(WTerm) + cr.reqs[0]->jobs.size() * sizeof(uint4);
     }
-    int threads = ctx->wg_threads ? ctx->wg_threads : WG;
+    int threads = wg_threads ? wg_threads : WG;
     threads = std::max(threads, cr.lp);
-    if (!ctx->wg_threads) {
+    if (!wg_threads) {
       while (threads > 64 && threads > cr.lp &&
              (paths_needed + threads / cr.lp - 1) / (threads / cr.lp) < 2 * cus)
         threads /= 2;
@@ -487,7 +509,7 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
     cr.lds = (size_t)per_wg * per_path;
     const size_t stage_limit =
         cr.mode == MODE_GRID ? limit : STAGED_LDS_LIMIT;
-    cr.staged = prog > 0 && cr.lds + prog <= stage_limit;
+    cr.staged = staging != 0 && prog > 0 && cr.lds + prog <= stage_limit;
     if (cr.staged) cr.lds += prog;
     break;
   }
@@ -838,6 +860,90 @@ int phc_stage_launch_info(phc_ctx* ctx, phc_staged* st, int32_t* info) {
 int phc_stage_destroy(phc_ctx* ctx, phc_staged* st) {
   if (ctx) (void)hipSetDevice(ctx->device);
   delete st;
+  return PHC_OK;
+}
+
+int phc_ctx_autotune(phc_ctx* ctx, const phc_system* sys,
+                     const phc_homotopy* hom, const phc_solve_options* opts,
+                     int64_t npaths, const double* start, int32_t* choice,
+                     double* best_ms) {
+  if (!ctx || !choice) return PHC_ERR_INVALID_INPUT;
+  if (npaths <= 0 || !start)
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "autotune needs start points");
+  HostReq probe;
+  int rc = validate_and_pack(ctx, sys, hom, opts, npaths, probe);
+  if (rc) return rc;
+  const int G = probe.G, N = probe.r.N;
+  const int64_t sample = std::min<int64_t>(npaths, 4096);
+  int saved[3];
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    saved[0] = ctx->mapping;
+    saved[1] = ctx->wg_threads;
+    saved[2] = ctx->staging;
+  }
+  struct Cand {
+    int mapping, wg, staging;
+  };
+  std::vector<Cand> cands;
+  const int maps[3] = {3, 1, 2};
+  for (int m : maps) {
+    if (m == 2 && !(G <= 8 && N <= 64 / G)) continue;
+    for (int wg : {64, 128, 256})
+      for (int stg : {-1, 0}) cands.push_back({m, wg, stg});
+  }
+  std::vector<std::array<int, 4>> seen;  // actual mode, threads, staged
+  double best = 1e300;
+  std::array<int, 3> pick = {0, 0, -1};
+  int measured = 0;
+  for (const Cand& c : cands) {
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      ctx->mapping = c.mapping;
+      ctx->wg_threads = c.wg;
+      ctx->staging = c.staging;
+    }
+    phc_staged* st = nullptr;
+    if (phc_stage_create(ctx, sys, hom, opts, sample, start, &st) != PHC_OK)
+      continue;
+    int32_t info[8] = {0};
+    phc_stage_launch_info(ctx, st, info);
+    const std::array<int, 4> id = {info[5], info[2], info[4], info[1]};
+    bool dup = false;
+    for (const auto& x : seen) dup = dup || x == id;
+    double ms = 0.0;
+    if (!dup) {
+      seen.push_back(id);
+      if (phc_stage_run(ctx, st) == PHC_OK &&          // code load, caches
+          phc_stage_run(ctx, st) == PHC_OK &&
+          phc_stage_elapsed(ctx, st, &ms) == PHC_OK) {
+        ++measured;
+        if (ms < best) {
+          best = ms;
+          const int mapping = info[5] == MODE_GRID ? 3 : (info[5] == MODE_SPLIT ? 2 : 1);
+          pick = {mapping, info[2], info[4] ? -1 : 0};
+        }
+      }
+    }
+    phc_stage_destroy(ctx, st);
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->mapping = saved[0];
+  ctx->wg_threads = saved[1];
+  ctx->staging = saved[2];
+  if (!measured) return fail(ctx, PHC_ERR_DEVICE, "no launch shape could be timed");
+  ctx->tuned[tune_key(probe)] = pick;
+  choice[0] = pick[0];
+  choice[1] = pick[1];
+  choice[2] = pick[2] == 0 ? 0 : 1;
+  if (best_ms) *best_ms = best;
+  return PHC_OK;
+}
+
+int phc_ctx_clear_tuning(phc_ctx* ctx) {
+  if (!ctx) return PHC_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->tuned.clear();
   return PHC_OK;
 }
 

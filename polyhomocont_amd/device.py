@@ -40,6 +40,8 @@ def lib():
         L.phc_ctx_sync.argtypes = [P]
         L.phc_ctx_counters.argtypes = [P, P]
         L.phc_ctx_set_mapping.argtypes = [P, C.c_int, C.c_int]
+        L.phc_ctx_autotune.argtypes = [P, P, P, P, i64, P, P, P]
+        L.phc_ctx_clear_tuning.argtypes = [P]
         L.phc_track_paths.argtypes = [P, P, P, P, i64, P, P]
         L.phc_stage_create.argtypes = [P, P, P, P, i64, P, C.POINTER(P)]
         L.phc_stage_run.argtypes = [P, P]
@@ -99,6 +101,25 @@ class Context:
 
     def lanes_per_path(self, N):
         return self.L.phc_ctx_lanes_per_path(self.h, N)
+
+    def autotune(self, packed, hom, opts, start):
+        """Times every launch shape on (a prefix of) this request and keeps
+        the fastest for later launches of the same shape; returns the choice
+        and the winner's kernel time."""
+        start = np.ascontiguousarray(start, np.complex128)
+        choice = np.zeros(4, np.int32)
+        ms = C.c_double(0.0)
+        rc = self.L.phc_ctx_autotune(self.h, C.byref(packed.c), C.byref(hom.c),
+                                     C.byref(opts), start.shape[0],
+                                     start.ctypes.data, choice.ctypes.data,
+                                     C.byref(ms))
+        self._check(rc, "phc_ctx_autotune")
+        return dict(mapping={1: "row", 2: "split", 3: "grid"}[int(choice[0])],
+                    threads=int(choice[1]), staged=bool(choice[2]),
+                    kernel_ms=ms.value)
+
+    def clear_tuning(self):
+        self._check(self.L.phc_ctx_clear_tuning(self.h), "phc_ctx_clear_tuning")
 
     def track_paths(self, packed, hom, opts, start):
         """Endpoint records for start points `start` [npaths][N]."""

@@ -207,3 +207,34 @@ def test_wide_systems_bit_exact(ctx, ref, mapping):
             assert_bit_exact(got, ref.endpoints(usys, opts, first, cnt), cnt)
     finally:
         ctx.set_mapping(0)
+
+
+def test_autotuned_launch_is_used_and_bit_exact(ctx, host):
+    """The autotuner's choice is applied to later launches of the same
+    shape, and the records stay the reference's."""
+    from polyhomocont_amd import device
+    G = golden()
+    for name in ("c1", "katsura4"):
+        usys = W.config_system(name) if name == "c1" else W.ref_test_systems()[name]
+        opts = abi.default_options()
+        consts = (G[name + "/gamma"][0], G[name + "/b"], G[name + "/patch"])
+        start = host.start_points(usys, opts.seed)
+        hom = abi.HomotopyData(*consts)
+        choice = ctx.autotune(usys.packed(), hom, opts, start)
+        print(f"[gpu] autotune {name}: {choice}", flush=True)
+        assert choice["mapping"] in ("row", "split", "grid")
+        assert choice["threads"] in (64, 128, 256)
+        st = device.Staged(ctx, usys.packed(), hom, opts, start)
+        try:
+            info = st.launch_info()
+            assert info["mapping"] == choice["mapping"]
+            assert info["threads"] == choice["threads"]
+            st.run()
+            got = st.fetch()
+        finally:
+            st.close()
+        n = len(start)
+        for f in FIELDS + ("eg_error", "t"):
+            assert np.array_equal(getattr(got, f)[:n], G[f"{name}/{f}"]), (name, f)
+        assert np.array_equal(got.z[:n], G[name + "/z"]), name
+    ctx.clear_tuning()
