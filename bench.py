@@ -98,7 +98,7 @@ def serving_benchmark(ctx, host_lib, count):
                      host_lib.start_points(usys, opts.seed)))
     paths = sum(len(r[3]) for r in reqs)
     log(f"serving: {len(reqs)} requests, {paths} paths prepared")
-    # warm both paths (code objects, trig tables)
+    # warm both paths (code objects, device buffers)
     ctx.track_paths(*reqs[0])
     srv = device.Server(ctx, max_resident_paths=16384,
                         hbm_budget_bytes=2 << 30, units_per_launch=64)

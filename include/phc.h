@@ -180,9 +180,8 @@ int phc_ctx_destroy(phc_ctx *ctx);
 int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
 
 /* Device counters of the last tracking call on this context:
- * out[0] arc angles not covered by the exact angle table (computed on the
- * device instead), out[1] linearization units executed (all groups),
- * out[2], out[3] reserved (0). */
+ * out[1] linearization units executed (all groups); out[0], out[2] and
+ * out[3] are reserved (0). */
 int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
 
 /* Lane mapping of later tracking calls. 0 chooses per launch: the G*G-lane

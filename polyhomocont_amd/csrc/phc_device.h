@@ -43,8 +43,6 @@ namespace phc {
 
 #define PHC_D __device__ __forceinline__
 
-constexpr int TRIG_BITS = 12;
-constexpr int TRIG_M = 1 << TRIG_BITS;
 constexpr int WG = 256;  // threads per workgroup
 
 // One term of a packed system: coefficient and the exponent of unknown j in
@@ -72,7 +70,6 @@ struct alignas(16) Req {
   const TermRec* terms;
   const struct WTerm* wterms;  // grid unit: compact terms
   const uint4* jobs;           // grid unit: [njobs][G G] job records
-  const cx* trig;   // [nsamples][TRIG_M + 1]: (cos, sin) of arc angles
   const cx* start;  // [npaths][N]
   int64_t npaths;
   // tracker_options
@@ -148,7 +145,7 @@ struct Launch {
   int32_t resume;           // load saved states at entry
   int32_t max_units;        // units per group per launch, 0 = unlimited
   unsigned int* active;     // groups still holding a path at exit
-  unsigned int* counters;   // [4]: trig fallbacks, units, be units, paths
+  unsigned int* counters;   // [4]: reserved, units, be units, paths
 };
 
 // ---------------------------------------------------------------- group ops
@@ -227,36 +224,17 @@ PHC_D void wave_lds_fence() {
 }
 
 // ------------------------------------------------------------------ segment
-// seg_t / seg_dt (src/tracker.f90:142-177). Lines are real (all lines the
-// tracker uses are), arcs look their angle up in the host-computed table
-// when s is a multiple of 2^-TRIG_BITS.
-PHC_D cx arc_cs(const Req& R, const PathState& S, double s,
-                unsigned int* fallbacks) {
-  double m = s * (double)TRIG_M;
-  if (m >= 0.0 && m <= (double)TRIG_M && m == rint(m)) {
-    const cx* tab = R.trig + (size_t)S.sj * (TRIG_M + 1);
-    return tab[(int)m];
-  }
-  double theta = S.sa + s * S.sdth;
-  if (fallbacks) atomicAdd(fallbacks, 1u);
-  return cr_cossin(theta);
-}
-
-PHC_D cx seg_t(const Req& R, const PathState& S, double s,
-               unsigned int* fb) {
+// seg_t (src/tracker.f90:142-162; seg_dt, :164-177, is formed where the
+// tangent needs it). Lines are real (all lines the
+// tracker uses are); arcs evaluate cos and sin of theta0 + s dtheta as
+// glibc's sincos does.
+PHC_D cx seg_t(const PathState& S, double s) {
   if (S.sk == 0) {
     if (s >= 1.0) return mk(S.sb, 0.0);
     return mk(S.sa + s * (S.sb - S.sa), 0.0);
   }
-  cx cs = arc_cs(R, S, s, fb);
+  const cx cs = libm_sincos(S.sa + s * S.sdth);
   return cscale(S.sb, cs);  // radius in sb
-}
-
-PHC_D cx seg_dt(const Req& R, const PathState& S, double s,
-                unsigned int* fb) {
-  if (S.sk == 0) return mk(S.sb - S.sa, 0.0);
-  cx t = seg_t(R, S, s, fb);
-  return mk(0.0 * t.re - S.sdth * t.im, 0.0 * t.im + S.sdth * t.re);
 }
 
 // -------------------------------------------------------------------- unit
@@ -627,7 +605,6 @@ struct Machine {
   const Req& R;
   PathState& S;
   const int lane;
-  unsigned int* fb;
   bool writer;  // the replica that stores results (split mode: replica 0)
 
   PHC_D double norm(cx v) const { return gnorm<G>(v, lane, R.N); }
@@ -781,7 +758,7 @@ struct Machine {
 
   PHC_D void main_done() {
     S.pstatus = S.sstatus;
-    S.pt = seg_t(R, S, S.s, fb).re;
+    S.pt = seg_t(S, S.s).re;
     S.nacc = S.sacc;
     S.nrej = S.srej;
     if (S.sstatus != PHC_PATH_SUCCESS || !R.use_endgame) {
@@ -1138,8 +1115,6 @@ __global__ void __launch_bounds__(WG)
     S.pc = PC_IDLE;
   }
   int units = 0;
-  // trig-table fallbacks are counted once per path (lane 0 of replica 0)
-  unsigned int* fb = (writer && lane == 0) ? L.counters : nullptr;
   for (;;) {
     if (S.pc == PC_IDLE) {
       if (L.max_units && units >= L.max_units) break;
@@ -1187,17 +1162,17 @@ __global__ void __launch_bounds__(WG)
         for (int i = (int)(threadIdx.x % LP); i < L.vmax; i += LP)
           wl.V[i] = mk(0.0, 0.0);
       }
-      Machine<G> M{R, S, lane, fb, writer};
+      Machine<G> M{R, S, lane, writer};
       M.start_path((lane < R.N) ? R.start[(size_t)S.path * R.N + lane]
                                 : mk(0.0, 0.0));
     } else {
       if (L.max_units && units >= L.max_units) break;
       const Req& R = STAGED ? *sreq : L.reqs[S.req];
-      Machine<G> M{R, S, lane, fb, writer};
+      Machine<G> M{R, S, lane, writer};
       cx t = mk(0.0, 0.0), dt = mk(0.0, 0.0);
       if (S.pc != PC_BE) {
         const double sv = (S.pc == PC_TAN) ? S.se : S.sn;
-        t = seg_t(R, S, sv, fb);
+        t = seg_t(S, sv);
         if (S.pc == PC_TAN)
           dt = (S.sk == 0) ? mk(S.sb - S.sa, 0.0)
                            : mk(0.0 * t.re - S.sdth * t.im,
@@ -1222,7 +1197,7 @@ __global__ void __launch_bounds__(WG)
         M.on_backward_error(be);
     }
     const Req& R = STAGED ? *sreq : L.reqs[S.req];
-    Machine<G> M{R, S, lane, fb, writer};
+    Machine<G> M{R, S, lane, writer};
     M.run();
   }
   if (L.slots) L.slots[(size_t)slot * G + lane] = S;

@@ -24,6 +24,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "phc_sincos_table.h"
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define PHC_HD __host__ __device__ __forceinline__
@@ -175,111 +177,115 @@ PHC_HD cx zpowi(cx base, int e) {
   return acc;
 }
 
-// ---- correctly rounded sin/cos ------------------------------------------
-// Double-double evaluation (Dekker products with FMA) of cos and sin for
-// |theta| < 2^20: Cody-Waite reduction by pi/2 with a three-part pi/2,
-// Taylor series to 2^-110 on |r| <= pi/4, one final rounding. The result
-// is the correctly rounded double except when the exact value lies within
-// ~2^-100 relative of a rounding midpoint.
-struct dd {
-  double hi, lo;
-};
-PHC_HD dd dd_two_sum(double a, double b) {
-  double s = a + b;
-  double bb = s - a;
-  double e = (a - (s - bb)) + (b - bb);
-  dd r;
-  r.hi = s;
-  r.lo = e;
-  return r;
-}
-PHC_HD dd dd_fast(double a, double b) {
-  double s = a + b;
-  dd r;
-  r.hi = s;
-  r.lo = b - (s - a);
-  return r;
-}
-PHC_HD dd dd_add(dd a, dd b) {
-  dd s = dd_two_sum(a.hi, b.hi);
-  dd t = dd_two_sum(a.lo, b.lo);
-  s.lo += t.hi;
-  s = dd_fast(s.hi, s.lo);
-  s.lo += t.lo;
-  return dd_fast(s.hi, s.lo);
-}
-PHC_HD dd dd_mul(dd a, dd b) {
-  double p = a.hi * b.hi;
-  double e = fma(a.hi, b.hi, -p);
-  e += a.hi * b.lo + a.lo * b.hi;
-  return dd_fast(p, e);
-}
-PHC_HD dd dd_mul_d(dd a, double b) {
-  double p = a.hi * b;
-  double e = fma(a.hi, b, -p);
-  e += a.lo * b;
-  return dd_fast(p, e);
+// ---- sin and cos as glibc's sincos computes them ------------------------
+// The reference's arc angles (src/tracker.f90:159) reach glibc 2.35's
+// sincos (flang -O2 merges the cos/sin pair into one call). Its dbl-64
+// implementation is the IBM Accurate Mathematical Library method: the
+// argument is reduced by pi/2 (hp0 + hp1 on 0.855 <= |x| < 2.426, a
+// four-part pi/2 beyond), sin and cos of the reduced x + dx come from the
+// table value at the nearest multiple of 1/128 combined with short
+// polynomials for the remainder, or from a Taylor polynomial for |x| <
+// 0.126. glibc's x86-64 sincos is built without FMA, and so is this. The
+// constants below are that method's published coefficients; the table is
+// phc_sincos_table.h. tests/test_math_host.py checks libm_sincos against
+// the host's sincos bit for bit on every branch.
+namespace sincos_detail {
+constexpr double BIG = 0x1.8p45;  // big + |x| rounds |x| to a multiple of 1/128
+constexpr double HP0 = 0x1.921fb54442d18p0, HP1 = 0x1.1a62633145c07p-54;
+constexpr double SN3 = -1.66666666666664880952546298448555E-01;
+constexpr double SN5 = 8.33333214285722277379541354343671E-03;
+constexpr double CS2 = 4.99999999999999999999950396842453E-01;
+constexpr double CS4 = -4.16666666666664434524222570944589E-02;
+constexpr double CS6 = 1.38888874007937613028114285595617E-03;
+constexpr double T1 = -0x1.5555555555555p-3, T2 = 0.0083333333333323288;
+constexpr double T3 = -0.00019841269834414642, T4 = 2.755729806860771e-06;
+constexpr double T5 = -2.502201484831839e-08;
+constexpr double HPINV = 0x1.45f306dc9c883p-1, TOINT = 0x1.8p52;
+constexpr double MP1 = 0x1.921fb58p0, MP2 = -0x1.dde973cp-27;
+constexpr double PP3 = -0x1.cb3b398p-55, PP4 = -0x1.d747f23e32ed7p-83;
+
+PHC_HD uint64_t bits_of(double x) {
+  uint64_t b;
+  __builtin_memcpy(&b, &x, sizeof b);
+  return b;
 }
 
-// sin (which = 0) and cos (which = 1) of a double-double r, |r| <= pi/4.
-PHC_HD dd dd_sincos_kernel(dd r, int which) {
-  dd r2 = dd_mul(r, r);
-  // term_k = (-1)^k r^(2k+which) / (2k+which)!
-  dd term;
-  if (which == 0) {
-    term = r;
-  } else {
-    term.hi = 1.0;
-    term.lo = 0.0;
-  }
-  dd sum = term;
-  int m = (which == 0) ? 1 : 0;  // power of the current term
-  for (int k = 1; k < 16; ++k) {
-    term = dd_mul(term, r2);
-    const double den = (double)((m + 1) * (m + 2));
-    // divide by den in double-double: q = term / den
-    double qh = term.hi / den;
-    double rem = fma(-qh, den, term.hi) + term.lo;
-    double ql = rem / den;
-    term.hi = -qh;
-    term.lo = -ql;
-    term = dd_fast(term.hi, term.lo);
-    m += 2;
-    sum = dd_add(sum, term);
-    if (fabs(term.hi) < 0x1p-112 * fabs(sum.hi)) break;
-  }
-  return sum;
+// sin(a + da) for |a| < 0.126
+PHC_HD double taylor_sin(double a, double da) {
+  const double xx = a * a;
+  const double p = (((T5 * xx + T4) * xx + T3) * xx + T2) * xx + T1;
+  const double t = (p * a - 0.5 * da) * xx + da;
+  return a + t;
 }
 
-// (cos theta, sin theta), each correctly rounded.
-PHC_HD_COLD cx cr_cossin(double theta) {
-  // pi/2 = P1 + P2 + P3 (53 + 53 + 53 bits)
-  const double P1 = 0x1.921fb54442d18p+0;
-  const double P2 = 0x1.1a62633145c07p-54;
-  const double P3 = -0x1.f1976b7ed8fbcp-110;
-  const double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
-  double kd = rint(theta * TWO_OVER_PI);
-  // r = theta - k P1 - k P2 - k P3 in double-double, products exact
-  double p1 = kd * P1;
-  double e1 = fma(kd, P1, -p1);
-  dd r = dd_two_sum(theta, -p1);
-  r = dd_add(r, dd_fast(-e1, 0.0));
-  double p2 = kd * P2;
-  double e2 = fma(kd, P2, -p2);
-  r = dd_add(r, dd_two_sum(-p2, -e2));
-  r = dd_add(r, dd_fast(-kd * P3, 0.0));
-  const int q = ((int)kd) & 3;
-  dd sn = dd_sincos_kernel(r, 0);
-  dd cs = dd_sincos_kernel(r, 1);
-  double s = sn.hi + sn.lo, c = cs.hi + cs.lo;
-  cx out;
-  switch (q) {
-    case 0: out = mk(c, s); break;
-    case 1: out = mk(-s, c); break;
-    case 2: out = mk(-c, -s); break;
-    default: out = mk(s, -c); break;
+// cos(x + dx)
+PHC_HD double cos_part(double x, double dx) {
+  if (x < 0) dx = -dx;
+  const double u = BIG + fabs(x);
+  x = fabs(x) - (u - BIG) + dx;
+  const double xx = x * x;
+  const double s = x + x * xx * (SN3 + xx * SN5);
+  const double c = xx * (CS2 + xx * (CS4 + xx * CS6));
+  const double* e = SINCOS_TABLE + 4 * (uint32_t)bits_of(u);
+  const double cor = (e[3] - s * e[1] - e[2] * c) - e[0] * s;
+  return e[2] + cor;
+}
+
+// sin(x + dx)
+PHC_HD double sin_part(double x, double dx) {
+  if (fabs(x) < 0.126) return taylor_sin(x, dx);
+  const double xold = x;
+  if (x <= 0) dx = -dx;
+  const double u = BIG + fabs(x);
+  x = fabs(x) - (u - BIG);
+  const double xx = x * x;
+  const double s = x + (dx + x * xx * (SN3 + xx * SN5));
+  const double c = x * dx + xx * (CS2 + xx * (CS4 + xx * CS6));
+  const double* e = SINCOS_TABLE + 4 * (uint32_t)bits_of(u);
+  const double cor = (e[1] + s * e[3] - e[0] * c) + e[2] * s;
+  return copysign(e[0] + cor, xold);
+}
+}  // namespace sincos_detail
+
+// (cos x, sin x) for finite |x| < 105414350 (arc angles lie in [0, 2 pi]).
+PHC_HD_COLD cx libm_sincos(double x) {
+  using namespace sincos_detail;
+  const uint32_t k = (uint32_t)(bits_of(x) >> 32) & 0x7fffffffu;
+  if (k < 0x3e400000u) return mk(1.0, x);  // |x| < 2^-27
+  if (k < 0x3feb6000u)                     // |x| < 0.855469
+    return mk(cos_part(x, 0.0), sin_part(x, 0.0));
+  if (k < 0x400368fdu) {  // |x| < 2.426265: x = +-(pi/2 - (a + da))
+    const double y = HP0 - fabs(x);
+    const double a = y + HP1;
+    const double da = (y - a) + HP1;
+    return mk(sin_part(a, da), copysign(cos_part(a, da), x));
   }
-  return out;
+  if (k >= 0x7ff00000u) return mk(x - x, x - x);
+  // x = n pi/2 + (a + da)
+  const double t = x * HPINV + TOINT;
+  const double xn = t - TOINT;
+  const double y = (x - xn * MP1) - xn * MP2;
+  const int n = (int)(bits_of(t) & 3u);
+  double t1 = xn * PP3;
+  const double t2 = y - t1;
+  double da = (y - t2) - t1;
+  t1 = xn * PP4;
+  double a = t2 - t1;
+  da += (t2 - a) - t1;
+  if (n == 1 || n == 2) {
+    a = -a;
+    da = -da;
+  }
+  const double sv = sin_part(a, da), cv = cos_part(a, da);
+  // with r = a + da: (cos x, sin x) = (cos r, sin r), (-sin r, cos r),
+  // (-cos r, -sin r), (sin r, -cos r) for n = 0..3; r is negated above for
+  // n = 1, 2, so sv and cv already carry those signs
+  switch (n) {
+    case 0: return mk(cv, sv);
+    case 1: return mk(sv, cv);
+    case 2: return mk(-cv, sv);
+    default: return mk(sv, -cv);
+  }
 }
 
 }  // namespace phc

@@ -41,7 +41,6 @@ struct phc_ctx {
   // shape (G, N, terms, log2 of the path count): mapping 1-3, workgroup
   // threads, staging (phc_ctx_autotune)
   std::map<std::array<int64_t, 4>, std::array<int, 3>> tuned;
-  std::map<int, cx*> trig;  // nsamples -> device table
   unsigned long long counters[4] = {0, 0, 0, 0};  // of the last run
   std::mutex mu;
   char err[512] = {0};
@@ -294,34 +293,6 @@ bool build_grid(HostReq& h, int G) {
   r.vstride = vstride;
   r.gfold = maxnnz | (maxp << 8) | (maxs << 16) | (fl << 24);
   return true;
-}
-
-// (cos, sin) of theta0_j + (m / 2^TRIG_BITS) dtheta with the host libm,
-// theta0_j = j dtheta: the angles of loop arcs (src/endgame.f90:362-363,
-// src/tracker.f90:158-159) at dyadic segment parameters.
-int get_trig(phc_ctx* ctx, int nsamples, cx** out) {
-  auto it = ctx->trig.find(nsamples);
-  if (it != ctx->trig.end()) {
-    *out = it->second;
-    return PHC_OK;
-  }
-  const double dth = 2.0 * 3.14159265358979323846 / (double)nsamples;
-  std::vector<cx> tab((size_t)nsamples * (TRIG_M + 1));
-  for (int j = 0; j < nsamples; ++j) {
-    const double th0 = (double)j * dth;
-    for (int m = 0; m <= TRIG_M; ++m) {
-      const double s = (double)m / (double)TRIG_M;
-      const double th = th0 + s * dth;
-      tab[(size_t)j * (TRIG_M + 1) + m] = mk(std::cos(th), std::sin(th));
-    }
-  }
-  cx* d = nullptr;
-  HIPCHK(ctx, hipMalloc(&d, tab.size() * sizeof(cx)));
-  HIPCHK(ctx, hipMemcpy(d, tab.data(), tab.size() * sizeof(cx),
-                        hipMemcpyHostToDevice));
-  ctx->trig[nsamples] = d;
-  *out = d;
-  return PHC_OK;
 }
 
 template <int G, int MODE>
@@ -598,10 +569,6 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
       h->r.o_egerr = (double*)arena_take(st->arena, (size_t)h->npaths * 8);
       h->r.o_egres = (double*)arena_take(st->arena, (size_t)h->npaths * 8);
       HIPCHK(ctx, hipMemsetAsync(h->r.o_done, 0, 4, ctx->stream));
-      if (h->r.use_endgame) {
-        int rc = get_trig(ctx, h->r.nsamples, (cx**)&h->r.trig);
-        if (rc) return rc;
-      }
       host_reqs.push_back(h->r);
       base.push_back(base.back() + h->npaths);
     }
@@ -670,8 +637,7 @@ int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
     HIPCHK(ctx, hipMemcpyAsync(c, cr.d_counters, sizeof(c),
                                hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->counters[0] += c[0];  // one lane of a path adds
-    ctx->counters[1] += c[1];                   // lane 0 of a group adds
+    ctx->counters[1] += c[1];  // lane 0 of a group adds
   }
   for (auto& up : st->reqs) {
     HostReq& h = *up;
@@ -757,7 +723,6 @@ int phc_ctx_create(int device, phc_ctx** out) {
 int phc_ctx_destroy(phc_ctx* ctx) {
   if (!ctx) return PHC_OK;
   (void)hipSetDevice(ctx->device);
-  for (auto& kv : ctx->trig) (void)hipFree(kv.second);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PHC_OK;

@@ -91,7 +91,7 @@ class Context:
     def counters(self):
         c = np.zeros(4, np.uint64)
         self.L.phc_ctx_counters(self.h, c.ctypes.data)
-        return dict(trig_fallbacks=int(c[0]), units=int(c[1]))
+        return dict(units=int(c[1]))
 
     def set_mapping(self, mapping=0, wg_threads=0):
         """0: chosen per launch, 1: path per G-lane group, 2: path per

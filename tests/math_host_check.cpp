@@ -3,13 +3,12 @@
 //   cdiv       vs compiler-rt __divdc3 (what flang emits for complex '/')
 //   cabs_      vs glibc hypot (what flang's abs(complex) reaches)
 //   zpowi      vs flang runtime _FortranAzpowi (complex ** integer)
-//   cr_cossin  vs libquadmath cosq/sinq rounded once to double
+//   libm_sincos vs glibc sincos (what flang -O2 emits for a cos/sin pair)
 // Inputs: random magnitudes over wide binades plus special values (zeros,
 // infinities, NaN, subnormals, values near the hypot scaling thresholds).
 // Prints one line per function: "<name> <checked> <mismatches>".
 #include <float.h>
 #include <math.h>
-#include <quadmath.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -100,16 +99,37 @@ int main(int argc, char** argv) {
   }
   printf("zpowi %ld %ld\n", cnt, bad);
 
-  // correctly rounded cos/sin on the arc-angle range [-8, 8]
+  // glibc sincos: the arc angles j dtheta + s dtheta (nsamples 1..64,
+  // dyadic and general s), uniform angles over [-8, 8], every branch's
+  // boundary neighbourhood and tiny arguments
   bad = cnt = 0;
-  for (long i = 0; i < n / 4; ++i) {
-    double th = (uniform() * 2.0 - 1.0) * 8.0;
-    if (i % 16 == 0) th = ldexp(uniform(), -(int)(next_u64() % 60));
-    cx v = phc::cr_cossin(th);
-    bad += !(same(v.re, (double)cosq((__float128)th)) &&
-             same(v.im, (double)sinq((__float128)th)));
+  auto check_sc = [&](double th) {
+    double s, c;
+    sincos(th, &s, &c);
+    cx v = phc::libm_sincos(th);
+    bad += !(same(v.re, c) && same(v.im, s));
     ++cnt;
+  };
+  for (long i = 0; i < n; ++i) {
+    const int ns = 1 + (int)(next_u64() % 64);
+    const double dth = 2.0 * 3.14159265358979323846 / (double)ns;
+    const double th0 = (double)(int)(next_u64() % (uint64_t)ns) * dth;
+    double sv = uniform();
+    if (i & 1) sv = ldexp((double)(next_u64() >> 11), -(int)(next_u64() % 54));
+    if (sv > 1.0) sv = ldexp(sv, -53);
+    check_sc(th0 + sv * dth);
+    check_sc((uniform() * 2.0 - 1.0) * 8.0);
   }
-  printf("cr_cossin %ld %ld\n", cnt, bad);
+  const double EDGES[] = {0x1p-27, 0.126, 0.855469, 2.426265, 1.5707963267948966,
+                          3.141592653589793, 4.71238898038469, 6.283185307179586};
+  for (double e : EDGES)
+    for (int k = -20000; k <= 20000; ++k) {
+      check_sc(e + k * 0x1p-52 * e);
+      check_sc(-(e + k * 0x1p-40 * e));
+    }
+  for (long i = 0; i < n / 8; ++i) check_sc(ldexp(uniform(), -(int)(next_u64() % 1000)));
+  check_sc(0.0);
+  check_sc(-0.0);
+  printf("libm_sincos %ld %ld\n", cnt, bad);
   return 0;
 }

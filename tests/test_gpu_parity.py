@@ -1,11 +1,12 @@
 """GPU tracker vs the oracle, path by path, through the C ABI.
 
 The device reproduces the reference's arithmetic operation by operation, so
-on these inputs the endpoint records are bit-identical: same status,
-winding number, end game status, accepted/rejected step counts and the
-same endpoint doubles. Hard cases (C3: endpoints at infinity with winding
-numbers up to 14) are held to the stated tolerance: classification fields
-equal, endpoints within 1e-10 relative.
+the endpoint records are bit-identical: same status, winding number, end
+game status, accepted/rejected step counts and the same endpoint doubles.
+That includes the hard cases (C3: endpoints at infinity with winding
+numbers up to 14, whose end game loops evaluate arcs at non-dyadic
+angles), where a one-bit difference in an arc's cos or sin would change
+the step sequence.
 """
 import os
 
@@ -69,8 +70,12 @@ def test_c2_sample_bit_exact(ctx, ref):
     assert_bit_exact(got, want, cnt)
 
 
-@pytest.mark.parametrize("name,first,count", [("c3v", 0, 96), ("c3", 200, 24)])
-def test_c3_sample_classification_exact(ctx, ref, name, first, count):
+# c3 paths 95 and 122 took another step sequence when the device's arc
+# cos/sin was correctly rounded, which differs from glibc's sincos in the
+# last bit for about 1 angle in 350.
+@pytest.mark.parametrize("name,first,count", [("c3v", 0, 96), ("c3", 0, 40),
+                                              ("c3", 95, 1), ("c3", 122, 1)])
+def test_c3_sample_bit_exact(ctx, ref, name, first, count):
     usys = W.config_system(name)
     opts = abi.default_options()
     g, b, a, ns = ref.homotopy(usys, opts.seed)
@@ -78,12 +83,27 @@ def test_c3_sample_classification_exact(ctx, ref, name, first, count):
     print(f"[gpu] {name} paths {first}..{first + count}", flush=True)
     got = run_device(ctx, usys, opts, (g, b, a), start)
     want = ref.endpoints(usys, opts, first, count)
-    for f in ("status", "winding", "eg_status"):
-        assert np.array_equal(getattr(got, f)[:count], getattr(want, f)[:count]), f
-    ok = want.status[:count] == 0
-    scale = np.maximum(np.abs(want.z[:count]).max(axis=1), 1e-300)
-    rel = np.abs(got.z[:count] - want.z[:count]).max(axis=1) / scale
-    assert rel[ok].max() <= 1e-10, rel[ok].max()
+    assert_bit_exact(got, want, count)
+
+
+def test_random_sparse_batch_bit_exact(ctx, ref):
+    """Random sparse systems with n = 3, 4, 5, 7, 8 (2302 paths) in one
+    batched call, every path against the reference."""
+    from polyhomocont_amd import host
+    H = host.library()
+    pool = W.batch_systems(91)
+    systems = [pool[i] for i in (0, 1, 16, 17, 41, 42, 68, 90)]
+    opts = abi.default_options()
+    items = []
+    for usys in systems:
+        g, b, a, ns = H.homotopy(usys, opts.seed)
+        items.append((usys.packed(), abi.HomotopyData(g, b, a), opts,
+                      H.start_points(usys, opts.seed)))
+    print(f"[gpu] batch of {len(items)} systems, "
+          f"{sum(u.bezout for u in systems)} paths", flush=True)
+    outs = ctx.track_batch(items)
+    for usys, got in zip(systems, outs):
+        assert_bit_exact(got, ref.endpoints(usys, opts), usys.bezout)
 
 
 def test_option_variants_match_reference(ctx, ref):

@@ -1,12 +1,12 @@
 """The device complex arithmetic (polyhomocont_amd/csrc/phc_math.h) compiled
 for the CPU and compared bit for bit with the routines the reference's flang
 build calls: compiler-rt __divdc3, glibc hypot, the flang runtime's
-complex ** integer, and correctly rounded cos/sin (libquadmath's binary128
-cosq/sinq rounded to double).
+complex ** integer, and glibc's sincos (the call flang -O2 makes for the
+cos/sin pair of an arc angle).
 
 The kernel includes the same header, so a difference here is a difference
-on the GPU. Needs g++, the ROCm LLVM runtime archives and libquadmath;
-skips when one of them is absent.
+on the GPU. Needs g++ and the ROCm LLVM runtime
+archives; skips when one of them is absent.
 """
 import glob
 import os
@@ -35,7 +35,7 @@ def checker(tmp_path_factory):
     cmd = [gxx, "-O2", "-std=c++17", "-ffp-contract=off",
            "-I", os.path.join(ROOT, "polyhomocont_amd", "csrc"),
            os.path.join(HERE, "math_host_check.cpp"), "-o", exe,
-           flang_rt, builtins, "-lquadmath", "-lm", "-lpthread"]
+           flang_rt, builtins, "-lm", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         pytest.fail("host build of phc_math.h failed:\n" + r.stderr[-3000:])
@@ -49,7 +49,7 @@ def checker(tmp_path_factory):
     return res
 
 
-@pytest.mark.parametrize("name", ["cdiv", "cabs", "zpowi", "cr_cossin"])
+@pytest.mark.parametrize("name", ["cdiv", "cabs", "zpowi", "libm_sincos"])
 def test_bit_exact_against_host_routine(checker, name):
     checked, bad = checker[name]
     assert checked > 100000
