@@ -190,6 +190,7 @@ def batch_benchmark(ctx, host_lib, count):
     t0 = time.perf_counter()
     outs = ctx.track_batch(items)
     dt = time.perf_counter() - t0
+    occ = ctx.counters()["occupancy"]
     ok = sum(int((o.status[:len(it[3])] == 0).sum()) for o, it in zip(outs, items))
     log(f"batch: {dt:.2f} s")
     return {"workload": f"{count} random sparse systems (n 3..8, degree 2..4, "
@@ -197,6 +198,7 @@ def batch_benchmark(ctx, host_lib, count):
             "systems": count, "paths": paths,
             "paths_per_s": round(paths / dt, 1), "seconds": round(dt, 3),
             "paths_success": ok,
+            "lane_occupancy_after_divergence": round(occ, 4),
             "includes": "host packing excluded; upload, kernels, download included"}
 
 
@@ -258,6 +260,7 @@ def main():
     wall = t1 - t0
     rec = st.fetch()
     info = st.launch_info()
+    occ = ctx.counters()["occupancy"]
 
     value = npaths * args.steps / wall
     flops_launch = int(flops.path_flops(packed, rec.nlin[:npaths],
@@ -301,9 +304,17 @@ def main():
                 info["mapping"]),
             "flops_per_launch": flops_launch,
             "avg_launch_ms": round(avg_s * 1e3, 4),
+            "hbm_gbps": round(traffic / avg_s / 1e9, 3) if traffic else None,
             "note": "FP64 vector ALU work (no MFMA); peak is the MI355X "
                     "FP64 figure",
         },
+        "path_divergence": {
+            "paths_per_wavefront": 64 // info["lanes_per_path"],
+            "lane_occupancy": round(occ, 4),
+            "loss": round(1.0 - occ, 4),
+            "definition": "1 - units run / (passes of each wavefront's "
+                          "tracking loop that ran a unit x paths per "
+                          "wavefront), device counters of the last timed launch"},
         "work": {"linearizations": int(rec.nlin[:npaths].sum()),
                  "backward_errors": int(rec.nbe[:npaths].sum()),
                  "steps_accepted": int(rec.naccepted[:npaths].sum()),

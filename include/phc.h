@@ -180,8 +180,12 @@ int phc_ctx_destroy(phc_ctx *ctx);
 int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
 
 /* Device counters of the last tracking call on this context:
- * out[1] linearization units executed (all groups); out[0], out[2] and
- * out[3] are reserved (0). */
+ * out[1] linearization units executed (all paths); out[2] unit slots
+ * offered: over all wavefronts, the passes of the tracking loop in which at
+ * least one of the wavefront's paths ran a unit, times the paths a
+ * wavefront holds. out[1] / out[2] is the lane occupancy left after path
+ * divergence (1 when a wavefront holds one path). out[0], out[3] are
+ * reserved (0). */
 int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
 
 /* Lane mapping of later tracking calls. 0 chooses per launch: the G*G-lane

@@ -145,7 +145,7 @@ struct Launch {
   int32_t resume;           // load saved states at entry
   int32_t max_units;        // units per group per launch, 0 = unlimited
   unsigned int* active;     // groups still holding a path at exit
-  unsigned int* counters;   // [4]: reserved, units, be units, paths
+  unsigned long long* counters;  // [4]: reserved, units, unit slots, reserved
 };
 
 // ---------------------------------------------------------------- group ops
@@ -1115,7 +1115,13 @@ __global__ void __launch_bounds__(WG)
     S.pc = PC_IDLE;
   }
   int units = 0;
+  // passes of this wavefront's loop in which at least one of its paths ran
+  // a unit: units / (passes * paths per wavefront) is the lane occupancy
+  // left after path divergence
+  int passes = 0;
   for (;;) {
+    if (__ballot(S.pc != PC_IDLE && !(L.max_units && units >= L.max_units)))
+      ++passes;
     if (S.pc == PC_IDLE) {
       if (L.max_units && units >= L.max_units) break;
       // pop the next queue index; -1 when the queue is empty
@@ -1202,7 +1208,13 @@ __global__ void __launch_bounds__(WG)
   }
   if (L.slots) L.slots[(size_t)slot * G + lane] = S;
   if (writer && S.pc != PC_IDLE && lane == 0 && L.active) atomicAdd(L.active, 1u);
-  if (writer && lane == 0 && L.counters) atomicAdd(&L.counters[1], (unsigned)units);
+  if (writer && lane == 0 && L.counters)
+    atomicAdd(&L.counters[1], (unsigned long long)units);
+  if (L.counters) {
+    for (int off = 32; off > 0; off >>= 1) passes = max(passes, __shfl_xor(passes, off));
+    if (threadIdx.x % 64 == 0)
+      atomicAdd(&L.counters[2], (unsigned long long)passes * (64 / LP));
+  }
 }
 
 }  // namespace phc

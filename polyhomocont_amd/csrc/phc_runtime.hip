@@ -356,7 +356,7 @@ struct ClassRun {
   Req* d_reqs = nullptr;
   int64_t* d_base = nullptr;
   unsigned long long* d_next = nullptr;
-  unsigned int* d_counters = nullptr;
+  unsigned long long* d_counters = nullptr;
   unsigned int* d_active = nullptr;
   PathState* d_slots = nullptr;
   int64_t total = 0;
@@ -581,7 +581,7 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
                                base.size() * sizeof(int64_t),
                                hipMemcpyHostToDevice, ctx->stream));
     cr.d_next = (unsigned long long*)arena_take(st->arena, 8);
-    cr.d_counters = (unsigned int*)arena_take(st->arena, 16);
+    cr.d_counters = (unsigned long long*)arena_take(st->arena, 32);
     cr.d_active = (unsigned int*)arena_take(st->arena, 4);
     cr.d_slots = (PathState*)arena_take(
         st->arena,
@@ -594,7 +594,7 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
 int run_classes(phc_ctx* ctx, phc_staged* st) {
   for (ClassRun& cr : st->classes) {
     HIPCHK(ctx, hipMemsetAsync(cr.d_next, 0, 8, ctx->stream));
-    HIPCHK(ctx, hipMemsetAsync(cr.d_counters, 0, 16, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(cr.d_counters, 0, 32, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(cr.d_active, 0, 4, ctx->stream));
     for (HostReq* h : cr.reqs)
       HIPCHK(ctx, hipMemsetAsync(h->r.o_done, 0, 4, ctx->stream));
@@ -633,11 +633,12 @@ int fetch_array(phc_ctx* ctx, T* host, const T* dev, size_t count) {
 int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
   for (int i = 0; i < 4; ++i) ctx->counters[i] = 0;
   for (ClassRun& cr : st->classes) {
-    unsigned int c[4];
+    unsigned long long c[4];
     HIPCHK(ctx, hipMemcpyAsync(c, cr.d_counters, sizeof(c),
                                hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->counters[1] += c[1];  // lane 0 of a group adds
+    ctx->counters[2] += c[2];  // lane 0 of a wavefront adds
   }
   for (auto& up : st->reqs) {
     HostReq& h = *up;

@@ -31,6 +31,8 @@
 // in the same order, so both units give identical bits.
 #pragma once
 
+#include <utility>
+
 namespace phc {
 
 // A term for the grid unit: coefficient; the LDS offsets e G + j of the
@@ -123,6 +125,44 @@ PHC_D cx pbcast_c(cx v, int src) {
 template <int LP>
 PHC_D cx pshfl_c(cx v, int src) {
   return mk(__shfl(v.re, src, LP), __shfl(v.im, src, LP));
+}
+
+// f(std::integral_constant<int, e>) for e = 0..E-1: the index of the
+// lane's slot arrays stays a compile-time constant even where the body
+// holds loops of its own (a plain loop around such a body is not always
+// unrolled, and a runtime index sends the arrays to scratch memory).
+template <typename F, int... I>
+PHC_D void static_for_seq(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int E, typename F>
+PHC_D void static_for(F&& f) {
+  static_for_seq(f, std::make_integer_sequence<int, E>{});
+}
+
+// Element i of a per-lane array whose index is not a compile-time constant:
+// selects in registers instead of letting the array fall back to scratch.
+template <int E>
+PHC_D cx pick(const cx (&a)[E], int i) {
+  cx v = a[0];
+#pragma unroll
+  for (int e = 1; e < E; ++e)
+    if (i == e) v = a[e];
+  return v;
+}
+template <int E>
+PHC_D bool pick(const bool (&a)[E], int i) {
+  bool v = a[0];
+#pragma unroll
+  for (int e = 1; e < E; ++e)
+    if (i == e) v = a[e];
+  return v;
+}
+template <int E>
+PHC_D void place(cx (&a)[E], int i, cx v) {
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    if (i == e) a[e] = v;
 }
 
 PHC_D int woff(uint64_t off, int a) { return (int)((off >> (8 * a)) & 255u); }
@@ -233,8 +273,8 @@ PHC_D void jac_general(const Req& R, const WTerm* __restrict__ wt,
   constexpr int FM = 8;
   const int maxp = gf_maxp(R.gfold), maxs = gf_maxs(R.gfold);
   const int M = R.njobs;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
+  static_for<E>([&](auto ec) {
+    constexpr int e = decltype(ec)::value;
     acc[e] = mk(0.0, 0.0);
     int fk = 0;
     for (int m = 0; m < M; ++m) {
@@ -273,7 +313,7 @@ PHC_D void jac_general(const Req& R, const WTerm* __restrict__ wt,
         acc[e] = cadd(acc[e], V[vbase[e] + fk]);
     }
     for (; fk < vcount[e]; ++fk) acc[e] = cadd(acc[e], V[vbase[e] + fk]);
-  }
+  });
 }
 
 // Factors that are exactly one: the reference multiplies by pw(0, j) = 1
@@ -350,8 +390,8 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   {
     const cx gt = cmul(R.gamma, t);
     const cx omt = mk(1.0 - t.re, 0.0 - t.im);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
+    static_for<E>([&](auto ec) {
+      constexpr int e = decltype(ec)::value;
       const int j = col[e];
       a[e] = mk(0.0, 0.0);
       rhs[e] = mk(0.0, 0.0);
@@ -388,7 +428,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
         rhs[e] = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
       }
       in[e] = lane < N && j < N;
-    }
+    });
   }
   if (mode == PC_BE) {
 #pragma unroll
@@ -415,39 +455,47 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   // logical position of this lane's row; pivot ties go to the smallest
   // position, as in the reference's first-maximum scan. Column k is in
   // group k % R, slot k / R.
+  // Step k works on slot ke = k / RG of group kr = k % RG; the steps of
+  // one slot share a compile-time ke, so the slot arrays stay in registers
+  // whichever way the step loop is compiled. The pivot lanes PL[k] (< 16)
+  // are packed 4 bits each.
   int pos = lane < N ? lane : (1 << 20);
-  int PL[G];
+  uint64_t plb = 0;
   int ok = 1;
+  static_for<E>([&](auto ec) {
+    constexpr int ke = decltype(ec)::value;
 #pragma unroll
-  for (int k = 0; k < G; ++k) {
-    if (k >= N) break;
-    const int kr = k % RG, ke = k / RG;
-    double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
-    int bp = pos, bl = lane;
-    gargmax<G>(bv, bp, bl);
-    bv = pbcast_d<LP>(bv, kr * G);
-    bp = pbcast_i<LP>(bp, kr * G);
-    bl = pbcast_i<LP>(bl, kr * G);
-    if (bv == 0.0) {
-      ok = 0;
-      break;
-    }
-    PL[k] = bl;
-    const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
-    int np = pos;
-    if (pos == k) np = bp;
-    if (lane == bl) np = k;
-    pos = np;
-    const cx l = cdiv(a[ke], piv);
-    if (in[ke] && rep == kr && pos > k) a[ke] = l;
-    const cx li = pshfl_c<LP>(a[ke], kr * G + lane);
+    for (int kr = 0; kr < RG; ++kr) {
+      const int k = ke * RG + kr;
+      if (!ok || k >= N) break;
+      double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
+      int bp = pos, bl = lane;
+      gargmax<G>(bv, bp, bl);
+      bv = pbcast_d<LP>(bv, kr * G);
+      bp = pbcast_i<LP>(bp, kr * G);
+      bl = pbcast_i<LP>(bl, kr * G);
+      if (bv == 0.0) {
+        ok = 0;
+        break;
+      }
+      plb |= (uint64_t)bl << (4 * k);
+      const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
+      int np = pos;
+      if (pos == k) np = bp;
+      if (lane == bl) np = k;
+      pos = np;
+      const cx l = cdiv(a[ke], piv);
+      if (in[ke] && rep == kr && pos > k) a[ke] = l;
+      const cx li = pshfl_c<LP>(a[ke], kr * G + lane);
+      // slots left of ke hold finished columns (col[e] < k)
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const cx pr = pshfl_c<LP>(a[e], rep * G + bl);
-      const cx upd = csub(a[e], cmul(li, pr));
-      if (in[e] && col[e] > k && pos > k) a[e] = upd;
+      for (int e = ke; e < E; ++e) {
+        const cx pr = pshfl_c<LP>(a[e], rep * G + bl);
+        const cx upd = csub(a[e], cmul(li, pr));
+        if (in[e] && col[e] > k && pos > k) a[e] = upd;
+      }
     }
-  }
+  });
   okout = ok;
   if (!ok) {
     xout = mk(0.0, 0.0);
@@ -481,38 +529,39 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     if (LP == 64) return pbcast_c<LP>(v, r);  // group 0, lane r
     return gshfl_c<G>(v, r);
   };
+  // arow and xs are indexed by compile-time constants only (static_for)
   cx sacc = mk(0.0, 0.0);
   bool sset = false;
-#pragma unroll
-  for (int k = 0; k < G; ++k) {
-    if (k >= N) break;
+  static_for<G>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k >= N) return;
     const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
-    const cx yk = rowcast(cand, PL[k]);
+    const cx yk = rowcast(cand, (int)((plb >> (4 * k)) & 15u));
     if (row && pos > k) {
       const cx p = cmul(arow[k], yk);
       sacc = sset ? cadd(sacc, p) : p;
       sset = true;
     }
-  }
+  });
   const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
   cx xs[G];
   cx xmine = mk(0.0, 0.0);
-#pragma unroll
-  for (int i = G - 1; i >= 0; --i) {
-    if (i < N) {
-      cx num = yfin;
-      if (i + 1 < N) {
-        cx sm = cmul(arow[i + 1], xs[i + 1]);
-#pragma unroll
-        for (int j = i + 2; j < G; ++j)
-          if (j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
-        num = csub(yfin, sm);
-      }
-      const cx cand = cdiv(num, arow[i]);
-      xs[i] = rowcast(cand, PL[i]);
-      if (lane == i) xmine = xs[i];
+  static_for<G>([&](auto ic) {
+    constexpr int i = G - 1 - decltype(ic)::value;
+    if (i >= N) return;
+    cx num = yfin;
+    if (i + 1 < N) {
+      cx sm = cmul(arow[(i + 1) % G], xs[(i + 1) % G]);
+      static_for<G>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if (j >= i + 2 && j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
+      });
+      num = csub(yfin, sm);
     }
-  }
+    const cx cand = cdiv(num, arow[i]);
+    xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
+    if (lane == i) xmine = xs[i];
+  });
   xout = xmine;
 }
 

@@ -91,7 +91,8 @@ class Context:
     def counters(self):
         c = np.zeros(4, np.uint64)
         self.L.phc_ctx_counters(self.h, c.ctypes.data)
-        return dict(units=int(c[1]))
+        occ = float(c[1]) / float(c[2]) if c[2] else None
+        return dict(units=int(c[1]), unit_slots=int(c[2]), occupancy=occ)
 
     def set_mapping(self, mapping=0, wg_threads=0):
         """0: chosen per launch, 1: path per G-lane group, 2: path per

@@ -201,6 +201,29 @@ def test_grid_mapping_is_default_for_small_systems(ctx, host):
             st.close()
 
 
+@pytest.mark.parametrize("mapping,paths_per_wave", [(1, 8), (2, 1), (3, 1)])
+def test_unit_counters(ctx, host, mapping, paths_per_wave):
+    """The device counts one unit per linearization or backward error of a
+    path, and offers units / (passes x paths per wavefront) slots: a
+    wavefront that holds one path never loses a slot to divergence."""
+    usys = W.config_system("c1")  # N = 5: 8-lane groups, 16 paths
+    opts = abi.default_options()
+    g, b, a, ns = host.homotopy(usys, opts.seed)
+    ctx.set_mapping(mapping)
+    try:
+        got = ctx.track_paths(host.pack(usys), abi.HomotopyData(g, b, a), opts,
+                              host.start_points(usys, opts.seed))
+    finally:
+        ctx.set_mapping(0)
+    c = ctx.counters()
+    assert c["units"] == int(got.nlin[:ns].sum()) + int(got.nbe[:ns].sum())
+    assert c["unit_slots"] >= c["units"]
+    if paths_per_wave == 1:
+        assert c["unit_slots"] == c["units"]
+    else:
+        assert 0.0 < c["occupancy"] < 1.0
+
+
 def _wide_systems():
     """Systems with 9..12 homogeneous unknowns (16-lane groups): a random
     sparse system with n = 8 and dense quadrics with n = 9 and n = 11."""
