@@ -127,6 +127,14 @@ PHC_D cx pshfl_c(cx v, int src) {
   return mk(__shfl(v.re, src, LP), __shfl(v.im, src, LP));
 }
 
+// Slots e of a lane that hold a column of an N-unknown system: columns
+// r + R e < N for some group r.
+template <int G>
+PHC_D int slots_used(int N) {
+  constexpr int R = GridShape<G>::R, E = GridShape<G>::E;
+  return E == 1 ? 1 : (N + R - 1) / R;
+}
+
 // f(std::integral_constant<int, e>) for e = 0..E-1: the index of the
 // lane's slot arrays stays a compile-time constant even where the body
 // holds loops of its own (a plain loop around such a body is not always
@@ -235,6 +243,7 @@ PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
   constexpr int LP = GridShape<G>::LP, E = GridShape<G>::E;
   constexpr int U = E == 1 ? 2 : 1;
   const int M = R.njobs;
+  const int eu = slots_used<G>(R.N);
 #pragma unroll
   for (int e = 0; e < E; ++e) acc[e] = mk(0.0, 0.0);
   for (int m = 0; m < M; m += U) {
@@ -243,12 +252,14 @@ PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
       uint4 job[E];
 #pragma unroll
       for (int e = 0; e < E; ++e)
-        job[e] = jobs[((size_t)(m + u) * LP + q) * E + e];
+        if (e < eu) job[e] = jobs[((size_t)(m + u) * LP + q) * E + e];
       cx d[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e) d[e] = job_product<FL>(job[e], wt, T);
+      for (int e = 0; e < E; ++e)
+        if (e < eu) d[e] = job_product<FL>(job[e], wt, T);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
+        if (e >= eu) continue;
         const int vb = vbase[e] + 4 * (m + u);
         const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
         acc[e] = cadd(acc[e], d[e]);
@@ -273,9 +284,11 @@ PHC_D void jac_general(const Req& R, const WTerm* __restrict__ wt,
   constexpr int FM = 8;
   const int maxp = gf_maxp(R.gfold), maxs = gf_maxs(R.gfold);
   const int M = R.njobs;
+  const int eu = slots_used<G>(R.N);
   static_for<E>([&](auto ec) {
     constexpr int e = decltype(ec)::value;
     acc[e] = mk(0.0, 0.0);
+    if (e >= eu) return;
     int fk = 0;
     for (int m = 0; m < M; ++m) {
       const uint4 job = jobs[((size_t)m * LP + q) * E + e];
@@ -459,6 +472,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   // one slot share a compile-time ke, so the slot arrays stay in registers
   // whichever way the step loop is compiled. The pivot lanes PL[k] (< 16)
   // are packed 4 bits each.
+  const int eu = slots_used<G>(N);
   int pos = lane < N ? lane : (1 << 20);
   uint64_t plb = 0;
   int ok = 1;
@@ -487,9 +501,11 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
       const cx l = cdiv(a[ke], piv);
       if (in[ke] && rep == kr && pos > k) a[ke] = l;
       const cx li = pshfl_c<LP>(a[ke], kr * G + lane);
-      // slots left of ke hold finished columns (col[e] < k)
+      // slots left of ke hold finished columns (col[e] < k), slots from
+      // eu on hold no column of the system
 #pragma unroll
       for (int e = ke; e < E; ++e) {
+        if (e >= eu) break;
         const cx pr = pshfl_c<LP>(a[e], rep * G + bl);
         const cx upd = csub(a[e], cmul(li, pr));
         if (in[e] && col[e] > k && pos > k) a[e] = upd;
