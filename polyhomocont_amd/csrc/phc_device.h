@@ -1120,8 +1120,6 @@ __global__ void __launch_bounds__(WG)
   // left after path divergence
   int passes = 0;
   for (;;) {
-    if (__ballot(S.pc != PC_IDLE && !(L.max_units && units >= L.max_units)))
-      ++passes;
     if (S.pc == PC_IDLE) {
       if (L.max_units && units >= L.max_units) break;
       // pop the next queue index; -1 when the queue is empty
@@ -1195,6 +1193,9 @@ __global__ void __launch_bounds__(WG)
         unit<G, SPLIT>(R, STAGED ? sterms : R.terms, lane, rep, S.pc, zl, t,
                        dt, gl, sl, x, ok, be);
       units++;
+      // this pass of the wavefront ran a unit: its lowest active lane counts
+      if ((int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)
+        ++passes;
       if (S.pc == PC_TAN)
         M.on_tangent(x, ok);
       else if (S.pc == PC_NEWT)
@@ -1211,7 +1212,7 @@ __global__ void __launch_bounds__(WG)
   if (writer && lane == 0 && L.counters)
     atomicAdd(&L.counters[1], (unsigned long long)units);
   if (L.counters) {
-    for (int off = 32; off > 0; off >>= 1) passes = max(passes, __shfl_xor(passes, off));
+    for (int off = 32; off > 0; off >>= 1) passes += __shfl_xor(passes, off);
     if (threadIdx.x % 64 == 0)
       atomicAdd(&L.counters[2], (unsigned long long)passes * (64 / LP));
   }

@@ -468,20 +468,15 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   // logical position of this lane's row; pivot ties go to the smallest
   // position, as in the reference's first-maximum scan. Column k is in
   // group k % R, slot k / R.
-  // Step k works on slot ke = k / RG of group kr = k % RG; the steps of
-  // one slot share a compile-time ke, so the slot arrays stay in registers
-  // whichever way the step loop is compiled. The pivot lanes PL[k] (< 16)
-  // are packed 4 bits each.
-  const int eu = slots_used<G>(N);
   int pos = lane < N ? lane : (1 << 20);
-  uint64_t plb = 0;
   int ok = 1;
-  static_for<E>([&](auto ec) {
-    constexpr int ke = decltype(ec)::value;
+  // pivot lanes of the steps: PL[k] (< 16) in bits 4k..4k+3
+  uint64_t plb = 0;
+  if constexpr (E == 1) {
 #pragma unroll
-    for (int kr = 0; kr < RG; ++kr) {
-      const int k = ke * RG + kr;
-      if (!ok || k >= N) break;
+    for (int k = 0; k < G; ++k) {
+      if (k >= N) break;
+      const int kr = k % RG, ke = k / RG;
       double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
       int bp = pos, bl = lane;
       gargmax<G>(bv, bp, bl);
@@ -501,17 +496,56 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
       const cx l = cdiv(a[ke], piv);
       if (in[ke] && rep == kr && pos > k) a[ke] = l;
       const cx li = pshfl_c<LP>(a[ke], kr * G + lane);
-      // slots left of ke hold finished columns (col[e] < k), slots from
-      // eu on hold no column of the system
 #pragma unroll
-      for (int e = ke; e < E; ++e) {
-        if (e >= eu) break;
+      for (int e = 0; e < E; ++e) {
         const cx pr = pshfl_c<LP>(a[e], rep * G + bl);
         const cx upd = csub(a[e], cmul(li, pr));
         if (in[e] && col[e] > k && pos > k) a[e] = upd;
       }
     }
-  });
+  } else {
+    // Step k works on slot ke = k / RG of group kr = k % RG; the steps of
+    // one slot share a compile-time ke, so the slot arrays stay in registers
+    // whichever way the step loop is compiled (the 16-lane unit's is not
+    // unrolled).
+    const int eu = slots_used<G>(N);
+    static_for<E>([&](auto ec) {
+      constexpr int ke = decltype(ec)::value;
+#pragma unroll
+      for (int kr = 0; kr < RG; ++kr) {
+        const int k = ke * RG + kr;
+        if (!ok || k >= N) break;
+        double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
+        int bp = pos, bl = lane;
+        gargmax<G>(bv, bp, bl);
+        bv = pbcast_d<LP>(bv, kr * G);
+        bp = pbcast_i<LP>(bp, kr * G);
+        bl = pbcast_i<LP>(bl, kr * G);
+        if (bv == 0.0) {
+          ok = 0;
+          break;
+        }
+        plb |= (uint64_t)bl << (4 * k);
+        const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
+        int np = pos;
+        if (pos == k) np = bp;
+        if (lane == bl) np = k;
+        pos = np;
+        const cx l = cdiv(a[ke], piv);
+        if (in[ke] && rep == kr && pos > k) a[ke] = l;
+        const cx li = pshfl_c<LP>(a[ke], kr * G + lane);
+        // slots left of ke hold finished columns (col[e] < k), slots from
+        // eu on hold no column of the system
+#pragma unroll
+        for (int e = ke; e < E; ++e) {
+          if (e >= eu) break;
+          const cx pr = pshfl_c<LP>(a[e], rep * G + bl);
+          const cx upd = csub(a[e], cmul(li, pr));
+          if (in[e] && col[e] > k && pos > k) a[e] = upd;
+        }
+      }
+    });
+  }
   okout = ok;
   if (!ok) {
     xout = mk(0.0, 0.0);
@@ -545,40 +579,76 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     if (LP == 64) return pbcast_c<LP>(v, r);  // group 0, lane r
     return gshfl_c<G>(v, r);
   };
-  // arow and xs are indexed by compile-time constants only (static_for)
-  cx sacc = mk(0.0, 0.0);
-  bool sset = false;
-  static_for<G>([&](auto kc) {
-    constexpr int k = decltype(kc)::value;
-    if (k >= N) return;
-    const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
-    const cx yk = rowcast(cand, (int)((plb >> (4 * k)) & 15u));
-    if (row && pos > k) {
-      const cx p = cmul(arow[k], yk);
-      sacc = sset ? cadd(sacc, p) : p;
-      sset = true;
+  if constexpr (E == 1) {
+    cx sacc = mk(0.0, 0.0);
+    bool sset = false;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      if (k >= N) break;
+      const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+      const cx yk = rowcast(cand, (int)((plb >> (4 * k)) & 15u));
+      if (row && pos > k) {
+        const cx p = cmul(arow[k], yk);
+        sacc = sset ? cadd(sacc, p) : p;
+        sset = true;
+      }
     }
-  });
-  const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
-  cx xs[G];
-  cx xmine = mk(0.0, 0.0);
-  static_for<G>([&](auto ic) {
-    constexpr int i = G - 1 - decltype(ic)::value;
-    if (i >= N) return;
-    cx num = yfin;
-    if (i + 1 < N) {
-      cx sm = cmul(arow[(i + 1) % G], xs[(i + 1) % G]);
-      static_for<G>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        if (j >= i + 2 && j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
-      });
-      num = csub(yfin, sm);
+    const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+    cx xs[G];
+    cx xmine = mk(0.0, 0.0);
+#pragma unroll
+    for (int i = G - 1; i >= 0; --i) {
+      if (i < N) {
+        cx num = yfin;
+        if (i + 1 < N) {
+          cx sm = cmul(arow[i + 1], xs[i + 1]);
+#pragma unroll
+          for (int j = i + 2; j < G; ++j)
+            if (j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
+          num = csub(yfin, sm);
+        }
+        const cx cand = cdiv(num, arow[i]);
+        xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
+        if (lane == i) xmine = xs[i];
+      }
     }
-    const cx cand = cdiv(num, arow[i]);
-    xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
-    if (lane == i) xmine = xs[i];
-  });
-  xout = xmine;
+    xout = xmine;
+  } else {
+    // arow and xs are indexed by compile-time constants only (static_for)
+    cx sacc = mk(0.0, 0.0);
+    bool sset = false;
+    static_for<G>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if (k >= N) return;
+      const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+      const cx yk = rowcast(cand, (int)((plb >> (4 * k)) & 15u));
+      if (row && pos > k) {
+        const cx p = cmul(arow[k], yk);
+        sacc = sset ? cadd(sacc, p) : p;
+        sset = true;
+      }
+    });
+    const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+    cx xs[G];
+    cx xmine = mk(0.0, 0.0);
+    static_for<G>([&](auto ic) {
+      constexpr int i = G - 1 - decltype(ic)::value;
+      if (i >= N) return;
+      cx num = yfin;
+      if (i + 1 < N) {
+        cx sm = cmul(arow[(i + 1) % G], xs[(i + 1) % G]);
+        static_for<G>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if (j >= i + 2 && j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
+        });
+        num = csub(yfin, sm);
+      }
+      const cx cand = cdiv(num, arow[i]);
+      xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
+      if (lane == i) xmine = xs[i];
+    });
+    xout = xmine;
+  }
 }
 
 }  // namespace phc
