@@ -72,6 +72,37 @@ def cpu_baseline(usys, opts, npaths, budget_s):
                        f"(1 host core, {dt:.1f} s)"), rec, count
 
 
+def _cpu_chunk(args):
+    """One slice of the all-core CPU baseline (runs in a worker process)."""
+    name, first, count = args
+    import oracle
+    from polyhomocont_amd import abi, workloads
+    ref = oracle.ref() or oracle.port()
+    ref.endpoints(workloads.config_system(name), abi.default_options(),
+                  first, count)
+    return count
+
+
+def cpu_all_cores(name, npaths, per_path_s, budget_s):
+    """The same reference tracker on every host core this process may use
+    (at most 16), paths handed out in chunks of 8 to worker processes.
+    Worker start-up is excluded: the pool is warmed before timing."""
+    import multiprocessing as mp
+    import oracle
+    kind = "reference" if oracle.ref() is not None else "port"
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    count = int(min(npaths, max(8 * cores, cores * budget_s / per_path_s)))
+    chunks = [(name, f, min(8, count - f)) for f in range(0, count, 8)]
+    with mp.get_context("spawn").Pool(cores) as pool:
+        pool.map(_cpu_chunk, [(name, 0, 1)] * cores, chunksize=1)
+        t0 = time.perf_counter()
+        done = sum(pool.imap_unordered(_cpu_chunk, chunks))
+        dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit="paths/s", cores=cores, kind=kind,
+                sample=f"paths 1..{count} of the same solve over {cores} "
+                       f"worker processes ({dt:.1f} s)")
+
+
 def percentiles(xs):
     import numpy as np
     xs = np.asarray(xs) * 1e3
@@ -325,6 +356,9 @@ def main():
         log("cpu baseline")
         cb, want, cnt = cpu_baseline(usys, opts, npaths, args.cpu_seconds)
         out["cpu_baseline"] = cb
+        log("cpu baseline, all cores")
+        out["cpu_baseline_all_cores"] = cpu_all_cores(
+            args.workload, npaths, 1.0 / cb["value"], args.cpu_seconds / 2)
         same = all(np.array_equal(getattr(rec, f)[:cnt], getattr(want, f)[:cnt])
                    for f in ("status", "winding", "eg_status"))
         scale = np.maximum(np.abs(want.z[:cnt]).max(axis=1), 1e-300)
