@@ -477,6 +477,9 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     for (int k = 0; k < G; ++k) {
       if (k >= N) break;
       const int kr = k % RG, ke = k / RG;
+      // column k of this lane's row, fetched while the pivot is searched:
+      // every group forms its row's multiplier itself
+      const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
       double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
       int bp = pos, bl = lane;
       gargmax<G>(bv, bp, bl);
@@ -489,17 +492,18 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
       }
       plb |= (uint64_t)bl << (4 * k);
       const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
+      cx pr[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) pr[e] = pshfl_c<LP>(a[e], rep * G + bl);
       int np = pos;
       if (pos == k) np = bp;
       if (lane == bl) np = k;
       pos = np;
-      const cx l = cdiv(a[ke], piv);
-      if (in[ke] && rep == kr && pos > k) a[ke] = l;
-      const cx li = pshfl_c<LP>(a[ke], kr * G + lane);
+      const cx li = cdiv(aik, piv);
+      if (in[ke] && rep == kr && pos > k) a[ke] = li;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const cx pr = pshfl_c<LP>(a[e], rep * G + bl);
-        const cx upd = csub(a[e], cmul(li, pr));
+        const cx upd = csub(a[e], cmul(li, pr[e]));
         if (in[e] && col[e] > k && pos > k) a[e] = upd;
       }
     }
@@ -515,6 +519,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
       for (int kr = 0; kr < RG; ++kr) {
         const int k = ke * RG + kr;
         if (!ok || k >= N) break;
+        const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
         double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
         int bp = pos, bl = lane;
         gargmax<G>(bv, bp, bl);
@@ -527,20 +532,22 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
         }
         plb |= (uint64_t)bl << (4 * k);
         const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
+        // slots left of ke hold finished columns (col[e] < k), slots from
+        // eu on hold no column of the system
+        cx pr[E];
+#pragma unroll
+        for (int e = ke; e < E; ++e)
+          if (e < eu) pr[e] = pshfl_c<LP>(a[e], rep * G + bl);
         int np = pos;
         if (pos == k) np = bp;
         if (lane == bl) np = k;
         pos = np;
-        const cx l = cdiv(a[ke], piv);
-        if (in[ke] && rep == kr && pos > k) a[ke] = l;
-        const cx li = pshfl_c<LP>(a[ke], kr * G + lane);
-        // slots left of ke hold finished columns (col[e] < k), slots from
-        // eu on hold no column of the system
+        const cx li = cdiv(aik, piv);
+        if (in[ke] && rep == kr && pos > k) a[ke] = li;
 #pragma unroll
         for (int e = ke; e < E; ++e) {
           if (e >= eu) break;
-          const cx pr = pshfl_c<LP>(a[e], rep * G + bl);
-          const cx upd = csub(a[e], cmul(li, pr));
+          const cx upd = csub(a[e], cmul(li, pr[e]));
           if (in[e] && col[e] > k && pos > k) a[e] = upd;
         }
       }
