@@ -329,6 +329,41 @@ PHC_D void jac_general(const Req& R, const WTerm* __restrict__ wt,
   });
 }
 
+// Pivot of one LU step among the candidates `cand` of the column's group
+// (src/linalg.f90:52-58: largest abs, first maximum in logical order),
+// broadcast from lane `src`. |a|^2 screens first: when exactly one
+// candidate has |a|^2 within 1e-12 of the largest, its hypot is strictly
+// the largest too (|a|^2 and hypot are both within a few units in the last
+// place), so the hypot values are not needed. Near ties, NaN entries and
+// |a|^2 outside the normal range fall back to comparing the hypot values
+// themselves.
+template <int G, int LP>
+PHC_D void pivot_search(bool cand, cx ak, int pos, int lane, int src,
+                        double& bv, int& bp, int& bl) {
+  const double sq = ak.re * ak.re + ak.im * ak.im;
+  double sv = cand ? sq : -1.0;
+  int sp = pos, sl = lane;
+  gargmax<G>(sv, sp, sl);
+  sv = pbcast_d<LP>(sv, src);
+  // a NaN candidate counts as a near tie: the hypot scan decides
+  unsigned long long near =
+      __ballot(cand && (sq >= sv * (1.0 - 1e-12) || sq != sq));
+  if (LP < 64) near = (near >> ((__lane_id() / LP) * LP)) & ((1ull << LP) - 1ull);
+  if (sv > 1e-280 && sv < 1e280 && __popcll(near) == 1) {
+    bv = sv;
+    bp = pbcast_i<LP>(sp, src);
+    bl = pbcast_i<LP>(sl, src);
+    return;
+  }
+  bv = cand ? cabs_(ak) : -1.0;
+  bp = pos;
+  bl = lane;
+  gargmax<G>(bv, bp, bl);
+  bv = pbcast_d<LP>(bv, src);
+  bp = pbcast_i<LP>(bp, src);
+  bl = pbcast_i<LP>(bl, src);
+}
+
 // Factors that are exactly one: the reference multiplies by pw(0, j) = 1
 // for every zero exponent (src/sparse.f90:262, :266); this unit multiplies
 // by 1 only to pad a lane's product to the length shared by the
@@ -480,12 +515,10 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
       // column k of this lane's row, fetched while the pivot is searched:
       // every group forms its row's multiplier itself
       const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
-      double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
-      int bp = pos, bl = lane;
-      gargmax<G>(bv, bp, bl);
-      bv = pbcast_d<LP>(bv, kr * G);
-      bp = pbcast_i<LP>(bp, kr * G);
-      bl = pbcast_i<LP>(bl, kr * G);
+      double bv;
+      int bp, bl;
+      pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, lane,
+                          kr * G, bv, bp, bl);
       if (bv == 0.0) {
         ok = 0;
         break;
@@ -520,12 +553,10 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
         const int k = ke * RG + kr;
         if (!ok || k >= N) break;
         const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
-        double bv = (in[ke] && rep == kr && pos >= k) ? cabs_(a[ke]) : -1.0;
-        int bp = pos, bl = lane;
-        gargmax<G>(bv, bp, bl);
-        bv = pbcast_d<LP>(bv, kr * G);
-        bp = pbcast_i<LP>(bp, kr * G);
-        bl = pbcast_i<LP>(bl, kr * G);
+        double bv;
+        int bp, bl;
+        pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, lane,
+                            kr * G, bv, bp, bl);
         if (bv == 0.0) {
           ok = 0;
           break;
