@@ -246,13 +246,35 @@ PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
   const int eu = slots_used<G>(R.N);
 #pragma unroll
   for (int e = 0; e < E; ++e) acc[e] = mk(0.0, 0.0);
+  // one slot per lane: the next step's job records are loaded while this
+  // step computes (with four slots the extra registers cost more than the
+  // latency they hide)
+  constexpr bool PREFETCH = E == 1;
+  uint4 jn[U][E];
+  if (PREFETCH) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (u < M) jn[u][0] = jobs[(size_t)u * LP + q];
+  }
   for (int m = 0; m < M; m += U) {
+    uint4 jc[U][E];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (PREFETCH)
+          jc[u][e] = jn[u][e];
+        else if (e < eu)
+          jc[u][e] = jobs[((size_t)(m + u) * LP + q) * E + e];
+    if (PREFETCH && m + U < M) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) jn[u][0] = jobs[(size_t)(m + U + u) * LP + q];
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       uint4 job[E];
 #pragma unroll
-      for (int e = 0; e < E; ++e)
-        if (e < eu) job[e] = jobs[((size_t)(m + u) * LP + q) * E + e];
+      for (int e = 0; e < E; ++e) job[e] = jc[u][e];
       cx d[E];
 #pragma unroll
       for (int e = 0; e < E; ++e)
