@@ -197,11 +197,11 @@ PHC_D void argmax_step(double& v, int& p, int& l) {
   const double ov = dpp_d<CTRL>(v);
   const int op = dpp_i<CTRL>(p);
   const int ol = dpp_i<CTRL>(l);
-  if (ov > v || (ov == v && op < p)) {
-    v = ov;
-    p = op;
-    l = ol;
-  }
+  // selects, not a branch: the comparison runs on every pivot step
+  const bool take = (ov > v) | ((ov == v) & (op < p));
+  v = take ? ov : v;
+  p = take ? op : p;
+  l = take ? ol : l;
 }
 template <int G>
 PHC_D void gargmax(double& v, int& p, int& l) {
