@@ -113,20 +113,19 @@ PHC_HD cx cdiv_apply(cx num, const cdiv_den& p) {
 PHC_HD cx cdiv(cx num, cx den) { return cdiv_apply(num, cdiv_prep(den)); }
 
 // Correction step of Borges' algorithm for 0 < ay <= ax, no overflow.
+// Both cases of the correction are formed and one is selected, so lanes
+// that fall in different cases do not serialize on the GPU.
 PHC_HD double hypot_kernel(double ax, double ay) {
-  double h = sqrt(ax * ax + ay * ay);
-  double t1, t2;
-  if (h <= 2.0 * ay) {
-    double delta = h - ay;
-    t1 = ax * (2.0 * delta - ax);
-    t2 = (delta - 2.0 * (ax - ay)) * delta;
-  } else {
-    double delta = h - ax;
-    t1 = 2.0 * delta * (ax - 2.0 * ay);
-    t2 = (4.0 * delta - ay) * ay + delta * delta;
-  }
-  h -= (t1 + t2) / (2.0 * h);
-  return h;
+  const double h = sqrt(ax * ax + ay * ay);
+  const double dn = h - ay;  // h <= 2 ay
+  const double t1n = ax * (2.0 * dn - ax);
+  const double t2n = (dn - 2.0 * (ax - ay)) * dn;
+  const double df = h - ax;  // otherwise
+  const double t1f = 2.0 * df * (ax - 2.0 * ay);
+  const double t2f = (4.0 * df - ay) * ay + df * df;
+  const bool near = h <= 2.0 * ay;
+  const double t1 = near ? t1n : t1f, t2 = near ? t2n : t2f;
+  return h - (t1 + t2) / (2.0 * h);
 }
 
 // hypot outside the common range: non-finite, huge or tiny operands.
@@ -154,8 +153,8 @@ PHC_HD double cabs_(cx a) {
   const double ay = x < y ? x : y;
   // common range: finite, ax <= 2^511, ay >= 2^-511
   if (!(ax <= 0x1p+511) || !(ay >= 0x1p-511)) return cabs_special(x, y);
-  if (ay <= ax * 0x1p-54) return ax + ay;
-  return hypot_kernel(ax, ay);
+  const double hk = hypot_kernel(ax, ay);
+  return (ay <= ax * 0x1p-54) ? ax + ay : hk;
 }
 
 // flang runtime complex ** integer (binary powering).
