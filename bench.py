@@ -111,7 +111,7 @@ def percentiles(xs):
             "mean": round(float(xs.mean()), 3)}
 
 
-def serving_benchmark(ctx, host_lib, count):
+def serving_benchmark(ctx, host_lib, count, units_per_launch=64):
     """The mixed request stream served by (a) the plain per-solve tracker,
     one request at a time, and (b) the continuous-batching engine:
     saturation throughput with every request queued at t = 0, then
@@ -132,7 +132,8 @@ def serving_benchmark(ctx, host_lib, count):
     # warm both paths (code objects, device buffers)
     ctx.track_paths(*reqs[0])
     srv = device.Server(ctx, max_resident_paths=16384,
-                        hbm_budget_bytes=2 << 30, units_per_launch=64)
+                        hbm_budget_bytes=2 << 30,
+                        units_per_launch=units_per_launch)
     t, _ = srv.submit(*reqs[0])
     srv.wait(t)
 
@@ -195,6 +196,7 @@ def serving_benchmark(ctx, host_lib, count):
                     "(n 3..8, degree 2..4, 5..40 terms) and dense (n 2..7, "
                     "degree 2..3), 8..4096 paths each",
         "paths": paths,
+        "units_per_launch": units_per_launch,
         "plain_paths_per_s": round(paths / plain_s, 1),
         "serving_paths_per_s": round(paths / serve_s, 1),
         "serving_speedup": round(plain_s / serve_s, 2),
@@ -242,6 +244,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-units", type=int, default=64,
+                    help="linearizations per path slot per engine launch")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=96,
@@ -371,7 +375,8 @@ def main():
     if args.batch_systems > 0:
         out["batch"] = batch_benchmark(ctx, H, args.batch_systems)
     if not args.no_serving:
-        out["serving"] = serving_benchmark(ctx, H, args.serving_requests)
+        out["serving"] = serving_benchmark(ctx, H, args.serving_requests,
+                                           args.serving_units)
     print(json.dumps(out), flush=True)
 
 
