@@ -394,7 +394,6 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
     // role of this lane in the ordered sums
     const bool jac_role = lane < n && rep < N;
     const bool f_role = lane == n && rep < n;
-    const int f_first = f_role ? R.first[rep] : 0;
     const int f_nt = f_role ? R.first[rep + 1] - R.first[rep] : 0;
     const unsigned below = (1u << rep) - 1u;
     cx acc = mk(0.0, 0.0);

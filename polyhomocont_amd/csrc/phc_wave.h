@@ -61,7 +61,6 @@ constexpr uint32_t WJOB_NONE = 0xFFFFFFFFu;
 // Request fields of the grid unit, packed in Req::gfold: largest number of
 // nonzero exponents of a term, of a job's prefix and suffix factors, and
 // the product length FL of the short layout (0: general layout).
-PHC_D int gf_maxnnz(int g) { return g & 255; }
 PHC_D int gf_maxp(int g) { return (g >> 8) & 255; }
 PHC_D int gf_maxs(int g) { return (g >> 16) & 255; }
 PHC_D int gf_fl(int g) { return (g >> 24) & 15; }
@@ -146,31 +145,6 @@ PHC_D void static_for_seq(F&& f, std::integer_sequence<int, I...>) {
 template <int E, typename F>
 PHC_D void static_for(F&& f) {
   static_for_seq(f, std::make_integer_sequence<int, E>{});
-}
-
-// Element i of a per-lane array whose index is not a compile-time constant:
-// selects in registers instead of letting the array fall back to scratch.
-template <int E>
-PHC_D cx pick(const cx (&a)[E], int i) {
-  cx v = a[0];
-#pragma unroll
-  for (int e = 1; e < E; ++e)
-    if (i == e) v = a[e];
-  return v;
-}
-template <int E>
-PHC_D bool pick(const bool (&a)[E], int i) {
-  bool v = a[0];
-#pragma unroll
-  for (int e = 1; e < E; ++e)
-    if (i == e) v = a[e];
-  return v;
-}
-template <int E>
-PHC_D void place(cx (&a)[E], int i, cx v) {
-#pragma unroll
-  for (int e = 0; e < E; ++e)
-    if (i == e) a[e] = v;
 }
 
 PHC_D int woff(uint64_t off, int a) { return (int)((off >> (8 * a)) & 255u); }
