@@ -602,6 +602,11 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
 #pragma unroll
   for (int j = 0; j < G; ++j)
     arow[j] = row ? W.A[lane * G + j] : mk(1.0, 0.0);
+  // The quotient of back-substitution step i is taken from the lane whose
+  // row is at logical position i, where arow[i] is that row's diagonal
+  // entry: its divisor scaling is prepared once, off the step chain.
+  const cdiv_den pdiag =
+      cdiv_prep(row ? W.A[lane * G + (pos < G ? pos : 0)] : mk(1.0, 0.0));
   cx bown = mk(0.0, 0.0);
   if (row) {
     // the slot that holds row `lane`'s right-hand side
@@ -641,7 +646,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
             if (j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
           num = csub(yfin, sm);
         }
-        const cx cand = cdiv(num, arow[i]);
+        const cx cand = cdiv_apply(num, pdiag);
         xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
         if (lane == i) xmine = xs[i];
       }
@@ -677,7 +682,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
         });
         num = csub(yfin, sm);
       }
-      const cx cand = cdiv(num, arow[i]);
+      const cx cand = cdiv_apply(num, pdiag);
       xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
       if (lane == i) xmine = xs[i];
     });
