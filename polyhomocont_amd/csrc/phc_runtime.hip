@@ -392,17 +392,20 @@ void* arena_take(Arena& a, size_t bytes) {
   return p;
 }
 
-// Lane mapping, launch shape and LDS of one lanes class. Rule (mapping 0):
-// the grid unit for G <= 8 when every request fits its tables, else one
-// path per wavefront with split terms when one path per group would leave
-// most SIMDs idle, else one path per group. Mappings 1-3 force row, split
-// or grid where the class allows it.
+// Key of the autotuner's cache: lanes class, unknowns, terms and the
+// binade of the path count.
 std::array<int64_t, 4> tune_key(const HostReq& h) {
   int64_t b = 0;
   while ((int64_t(1) << (b + 1)) <= std::max<int64_t>(h.npaths, 1)) ++b;
   return {h.G, h.r.N, h.r.nterms, b};
 }
 
+// Lane mapping, launch shape and LDS of one lanes class. A single-request
+// class takes the autotuner's cached choice when there is one. Otherwise
+// (mapping 0): the grid unit when every request fits its tables, else one
+// path per wavefront with split terms when one path per group would leave
+// most SIMDs idle, else one path per group. Mappings 1-3 force row, split
+// or grid where the class allows it.
 void plan_class(phc_ctx* ctx, ClassRun& cr) {
   const int G = cr.G;
   int mapping = ctx->mapping, wg_threads = ctx->wg_threads,
