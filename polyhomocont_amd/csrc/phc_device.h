@@ -1,13 +1,16 @@
 // phc_device.h -- the path tracker on the device.
 //
-// Mapping. One path is owned by a group of G lanes (G = 4, 8 or 16, a
-// power of two >= N = n + 1). Lane r of the group owns row r of every
-// N x N system and component r of every N-vector. Scalars of the path
-// state machine are replicated in all lanes of the group and updated
-// identically, so control flow is uniform inside a group. A wavefront holds
-// 64/G paths; a workgroup holds 256/G.
+// Mapping. A path is owned by a group of G lanes (G = 4, 8 or 16, a power
+// of two >= N = n + 1). Lane r of the group owns component r of every
+// N-vector. Scalars of the path state machine are replicated in all lanes
+// of the group and updated identically, so control flow is uniform inside
+// a group. The linearization unit either runs on the group itself, row r
+// in lane r (`unit` below, 64/G paths per wavefront), or on a grid of
+// min(G*G, 64) lanes of which the group is one replica (`unit_wave`,
+// phc_wave.h, the default; see the MODE list at the kernel).
 //
-// Linearization unit (the only heavy work): at a point zq and parameter t
+// Row unit (the grid unit does the same work with an entry per lane): at a
+// point zq and parameter t
 //   1. lane j writes the powers zq_j^e, e = 1..maxexp, to the group's table
 //      in LDS (src/sparse.f90:251-255);
 //   2. lane i < n walks the terms of equation i and accumulates F^h_i and
