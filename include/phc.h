@@ -175,8 +175,8 @@ typedef struct phc_ctx phc_ctx;
 int phc_ctx_create(int device, phc_ctx **ctx);
 int phc_ctx_destroy(phc_ctx *ctx);
 
-/* Lanes per path chosen for systems with N unknowns (4, 8 or 16), after
- * autotuning if phc_autotune ran on this context. */
+/* Lanes per group (4, 8 or 16) for systems with N unknowns: the power of
+ * two >= N that sizes the path's state. */
 int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
 
 /* Device counters of the last tracking call on this context:
@@ -188,13 +188,16 @@ int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
  * reserved (0). */
 int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
 
-/* Lane mapping of later tracking calls. 0 chooses per launch: the G*G-lane
- * grid for N <= 8 (a path's linear system one entry per lane, its
- * evaluation spread over all lanes), else one path per G-lane group.
- * 1 forces one path per G-lane group (row per lane), 2 one path per
- * wavefront with the terms of each equation split across 64/G replicas
- * where N <= 64/G, 3 the grid where N <= 8. wg_threads: 0 (chosen per
- * launch), 64, 128 or 256. Results do not depend on the mapping. */
+/* Lane mapping of later tracking calls. 0 chooses per launch: an
+ * autotuned choice (phc_ctx_autotune) for single-request launches of a
+ * tuned shape, else the min(G*G, 64)-lane grid (a path's linear system one
+ * entry per lane, up to four per lane for G = 16, its evaluation spread
+ * over all lanes) when the request fits the grid tables, else one path per
+ * G-lane group. 1 forces one path per G-lane group (row per lane), 2 one
+ * path per wavefront with the terms of each equation split across 64/G
+ * replicas where N <= 64/G, 3 the grid where the tables fit. wg_threads: 0
+ * (chosen per launch), 64, 128 or 256. Results do not depend on the
+ * mapping. */
 int phc_ctx_set_mapping(phc_ctx *ctx, int mapping, int wg_threads);
 
 /* Last error message of the context (static storage, never NULL). */

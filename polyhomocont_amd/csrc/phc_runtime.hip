@@ -30,7 +30,6 @@ struct phc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
-  int lanes_override[PHC_MAX_UNKNOWNS + 1] = {0};
   int wg_threads = 0;  // 0: choose per launch
   // 0: by rule, 1: one path per G-lane group, 2: one path per wavefront
   // with split terms, 3: one path per G*G-lane grid
@@ -66,9 +65,7 @@ int fail(phc_ctx* ctx, int code, const char* fmt, ...) {
                   hipGetErrorString(e_));                                  \
   } while (0)
 
-int lanes_for(const phc_ctx* ctx, int N) {
-  if (ctx && N >= 0 && N <= PHC_MAX_UNKNOWNS && ctx->lanes_override[N])
-    return ctx->lanes_override[N];
+int lanes_for(const phc_ctx*, int N) {
   if (N <= 4) return 4;
   if (N <= 8) return 8;
   return 16;
