@@ -248,7 +248,7 @@ def main():
                     help="linearizations per path slot per engine launch")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--batch-systems", type=int, default=96,
+    ap.add_argument("--batch-systems", type=int, default=512,
                     help="systems of the batched multi-system launch (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
