@@ -223,8 +223,10 @@ int phc_stage_run(phc_ctx *ctx, phc_staged *st);
 int phc_stage_fetch(phc_ctx *ctx, phc_staged *st, phc_endpoints *out);
 int phc_stage_destroy(phc_ctx *ctx, phc_staged *st);
 /* Device time (ms) of the tracking kernels of the last phc_stage_run,
- * measured with HIP events recorded on the context stream around each
- * launch; waits for the run to finish. */
+ * measured with HIP events: from the point of the context stream where
+ * every lanes class starts (the classes of one run are launched together
+ * on their own streams) to the end of the last class; waits for the run
+ * to finish. */
 int phc_stage_elapsed(phc_ctx *ctx, phc_staged *st, double *ms);
 /* Launch shape of the first lanes class (info has room for 8 values):
  * info[0] group size G (a power of two >= N), [1] workgroups, [2] threads
