@@ -191,6 +191,9 @@ def serving_benchmark(ctx, host_lib, count, units_per_launch=64):
             srv.release(t)
     srv.close()
     log("serving: engine latency run done")
+    fe_s, fe_same, fe_bytes = frontend_saturation(
+        ctx, host_lib, systems, plain_out, units_per_launch)
+    log(f"serving: front end saturation {fe_s:.2f} s")
     return {
         "workload": f"{len(reqs)} requests, alternately random sparse "
                     "(n 3..8, degree 2..4, 5..40 terms) and dense (n 2..7, "
@@ -204,7 +207,57 @@ def serving_benchmark(ctx, host_lib, count, units_per_launch=64):
         "qps": round(lam, 3),
         "latency_ms": {"serving": percentiles(lat_serve),
                        "plain": percentiles(lat_plain)},
+        "front_end": {
+            "paths_per_s": round(paths / fe_s, 1),
+            "records_identical": bool(fe_same),
+            "request_bytes": fe_bytes,
+            "includes": "request frames over TCP on 127.0.0.1, decoding, "
+                        "building each system and drawing its start points "
+                        "on the front end's thread, the engine, reply "
+                        "frames"},
     }
+
+
+def frontend_saturation(ctx, host_lib, systems, plain_out, units_per_launch):
+    """All requests sent at once as frames to the front end (own thread,
+    own engine) by one client; returns (seconds until the last reply,
+    records equal to the plain tracker's, request bytes)."""
+    import threading
+    import numpy as np
+    from polyhomocont_amd import abi, device, frontend, wire
+
+    frames = [wire.encode_request(u, abi.default_options(), i)
+              for i, u in enumerate(systems)]
+    srv = device.Server(ctx, max_resident_paths=16384,
+                        hbm_budget_bytes=2 << 30,
+                        units_per_launch=units_per_launch)
+    fe = frontend.FrontEnd(srv, host_lib)
+    stop = threading.Event()
+    th = threading.Thread(target=fe.serve, args=(stop,), daemon=True)
+    th.start()
+    got = {}
+    try:
+        cl = frontend.Client(fe.address)
+        t0 = time.perf_counter()
+        for f in frames:
+            wire.send_frame(cl.sock, f)
+        for _ in frames:
+            rid, rec, err, msg = cl.receive()
+            if err:
+                raise RuntimeError(f"front end rejected request {rid}: {msg}")
+            got[rid] = rec
+        dt = time.perf_counter() - t0
+        cl.close()
+    finally:
+        stop.set()
+        th.join(timeout=60)
+        fe.close()
+        srv.close()
+    same = all(np.array_equal(got[i].z[:got[i].npaths], p.z[:got[i].npaths])
+               and np.array_equal(got[i].status[:got[i].npaths],
+                                  p.status[:got[i].npaths])
+               for i, p in enumerate(plain_out))
+    return dt, same, sum(len(f) for f in frames)
 
 
 def batch_benchmark(ctx, host_lib, count):

@@ -6,6 +6,8 @@
                             the C-callable layer of include/polyhomocont.h),
                             flang, linked against libphc_hip.so
   lib/mod/*.mod             Fortran module files for Fortran callers
+  lib/libphc_wire.so        request/endpoint frames of a serving front end
+                            (include/phc_wire.h), host C++ built with g++
 
 Every call compiles from source (no up-to-date checks), so a build is always
 the current tree.
@@ -102,8 +104,19 @@ def build_oracle():
         _run(["make", "-s", "-C", odir, f"REF={ref}", *targets])
 
 
+def build_wire():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libphc_wire.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", "-Wextra", "-shared",
+          os.path.join(CSRC, "phc_wire.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all():
     build_hip()
+    build_wire()
     build_fortran()
     build_oracle()
 

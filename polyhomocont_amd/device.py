@@ -17,7 +17,11 @@ _lib = None
 
 
 class DeviceError(RuntimeError):
-    pass
+    """A failed library call; `code` is its PHC_ERR_* return value."""
+
+    def __init__(self, message, code=-1):
+        super().__init__(message)
+        self.code = code
 
 
 def lib():
@@ -86,7 +90,7 @@ class Context:
     def _check(self, rc, what):
         if rc != 0:
             msg = self.L.phc_ctx_last_error(self.h).decode(errors="replace")
-            raise DeviceError(f"{what} failed with {rc}: {msg}")
+            raise DeviceError(f"{what} failed with {rc}: {msg}", rc)
 
     def counters(self):
         c = np.zeros(4, np.uint64)

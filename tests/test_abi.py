@@ -32,6 +32,15 @@ def test_host_library_exports_header():
     assert not missing, missing
 
 
+def test_wire_library_exports_header():
+    from polyhomocont_amd import wire
+    lib = wire.lib()
+    names = declared("phc_wire.h")
+    assert "phc_wire_decode_request" in names and len(names) == 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
 def test_default_options_match_reference_defaults():
     from polyhomocont_amd import abi, device
     o = abi.SolveOptions()
