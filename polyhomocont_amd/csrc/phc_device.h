@@ -1086,7 +1086,7 @@ __global__ void __launch_bounds__(WG)
     base = staged_req_bytes() + (size_t)nt * sizeof(TermRec);
     if (GRID) {
       sjobs = (uint4*)(lds_raw + base);
-      const int nj = L.reqs[0].njobs * LP * GridShape<G>::E;
+      const int nj = job_records<G>(L.reqs[0]);
       const uint4* jsrc = L.reqs[0].jobs;
       for (int i = threadIdx.x; i < nj; i += blockDim.x) sjobs[i] = jsrc[i];
       base += (size_t)nj * sizeof(uint4);

@@ -193,6 +193,102 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
   return PHC_OK;
 }
 
+// One derivative job of the grid unit: term k, exponent e of its variable,
+// the offset of pw(e-1, j) and the offsets of the term's other factors.
+struct GridJob {
+  int k, e, base, np, ns;
+  int others[8];  // offsets of the other nonzero factors, ascending j
+};
+
+// Short-layout record of a job (the layout is described at WTerm).
+uint4 short_record(const GridJob& jb) {
+  uint4 rec = make_uint4((uint32_t)jb.k | ((uint32_t)jb.e << 20), 0u, 0u, 0u);
+  rec.y = (uint32_t)jb.base;
+  for (int b2 = 0; b2 < jb.np; ++b2)
+    rec.y |= (uint32_t)jb.others[b2] << (8 * (1 + b2));
+  // suffix in right-fold order: last factor first
+  for (int b2 = 0; b2 < jb.ns; ++b2)
+    rec.z |= (uint32_t)jb.others[jb.np + jb.ns - 1 - b2] << (8 * b2);
+  return rec;
+}
+
+// Run layout of the 16-lane grid (short layout only). Each sum of the
+// unit -- Jacobian entry (i, j) with its jobs in term order, or F_i with
+// one step per four term values -- is a run of consecutive steps on one
+// lane; the runs are dealt longest first, each to the lane with the fewest
+// steps so far, so a lane holds one or more whole runs. Record w: bits
+// 0..15 the V index of the step's four term values (the zero row n vs for
+// derivative steps), bits 16..30 the run's sum index (i N + j, or n N + i
+// for F_i) and bit 31 on the run's last step, where the lane stores the
+// sum. Table [step][lane], the step count even (two steps per turn).
+bool build_runs(HostReq& h, int G, std::vector<WTerm>& wt,
+                const std::vector<std::vector<GridJob>>& lists, int maxterms,
+                int maxnnz, int maxp, int maxs, int fl) {
+  Req& r = h.r;
+  const int N = r.N, n = r.n, nt = (int)h.terms.size();
+  const int LP = std::min(G * G, 64), RG = LP / G, E = G / RG;
+  const int vs = 4 * std::max(1, (maxterms + 3) / 4);
+  const int zero_row = n * vs;
+  if (zero_row + vs > 65535 || n * N + n > std::min(G * G, 1 << 15))
+    return false;
+  struct Run {
+    int id, len, i, j;  // j == -1: F_i
+  };
+  std::vector<Run> runs;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < N; ++j) {
+      const int len = (int)lists[((j % RG) * G + i) * E + j / RG].size();
+      runs.push_back({i * N + j, std::max(len, 1), i, j});
+    }
+    const int nti = r.first[i + 1] - r.first[i];
+    runs.push_back({n * N + i, std::max(1, (nti + 3) / 4), i, -1});
+  }
+  std::stable_sort(runs.begin(), runs.end(),
+                   [](const Run& a, const Run& b) { return a.len > b.len; });
+  std::vector<int> load(LP, 0);
+  std::vector<std::vector<int>> dealt(LP);
+  for (int x = 0; x < (int)runs.size(); ++x) {
+    int best = 0;
+    for (int q = 1; q < LP; ++q)
+      if (load[q] < load[best]) best = q;
+    dealt[best].push_back(x);
+    load[best] += runs[x].len;
+  }
+  size_t M = (size_t)*std::max_element(load.begin(), load.end());
+  M += M & 1;
+  for (int i = 0; i < n; ++i)
+    for (int k = r.first[i]; k < r.first[i + 1]; ++k)
+      wt[k].slot = (uint32_t)(i * vs + (k - r.first[i]));
+  wt[nt].slot = (uint32_t)zero_row;  // never written: not a real term
+  // padding steps: the zero-coefficient term, factors pw(0, 0) = 1
+  const uint4 pad = make_uint4((uint32_t)nt | (1u << 20), 0u, 0u,
+                               (uint32_t)zero_row);
+  h.jobs.assign(M * LP, pad);
+  for (int q = 0; q < LP; ++q) {
+    size_t m = 0;
+    for (int x : dealt[q]) {
+      const Run& ru = runs[x];
+      for (int s = 0; s < ru.len; ++s, ++m) {
+        uint4 rec = pad;
+        if (ru.j < 0) {
+          rec.w = (uint32_t)(ru.i * vs + 4 * s);
+        } else {
+          const auto& l = lists[((ru.j % RG) * G + ru.i) * E + ru.j / RG];
+          if (s < (int)l.size()) rec = short_record(l[s]);
+          rec.w = (uint32_t)zero_row;
+        }
+        if (s == ru.len - 1) rec.w |= ((uint32_t)ru.id << 16) | (1u << 31);
+        h.jobs[m * LP + q] = rec;
+      }
+    }
+  }
+  h.wterms.swap(wt);
+  r.njobs = (int32_t)M;
+  r.vstride = vs;
+  r.gfold = maxnnz | (maxp << 8) | (maxs << 16) | (fl << 24) | (1 << 28);
+  return true;
+}
+
 // Tables of the grid unit (phc_wave.h) for G*G lanes per path: compact
 // terms (plus a zero-coefficient term at index nterms) with the LDS offsets
 // e G + j of their nonzero factors and their V slots, and per lane (i, j)
@@ -205,10 +301,7 @@ bool build_grid(HostReq& h, int G) {
   const int LP = std::min(G * G, 64), RG = LP / G, E = G / RG;
   const int nt = (int)h.terms.size();
   if (N > G || nt + 1 >= (1 << 20)) return false;
-  struct Job {
-    int k, e, base, np, ns;
-    int others[8];  // offsets of the other nonzero factors, ascending j
-  };
+  using Job = GridJob;
   std::vector<WTerm> wt(nt + 1);
   memset(wt.data(), 0, wt.size() * sizeof(WTerm));
   std::vector<std::vector<Job>> lists(LP * E);  // slot q E + e
@@ -254,6 +347,8 @@ bool build_grid(HostReq& h, int G) {
   }
   // short layout: every term has <= 4 nonzero exponents
   const int fl = maxnnz <= 4 ? std::max(1, std::max(maxp, maxs)) : 0;
+  if (fl && E > 1)
+    return build_runs(h, G, wt, lists, maxterms, maxnnz, maxp, maxs, fl);
   size_t M = 0;
   for (const auto& l : lists) M = std::max(M, l.size());
   M = std::max(M, (size_t)((maxterms + 3) / 4));
@@ -271,12 +366,7 @@ bool build_grid(HostReq& h, int G) {
       const Job& jb = lists[qq][m];
       uint4 rec = make_uint4((uint32_t)jb.k | ((uint32_t)jb.e << 20), 0u, 0u, 0u);
       if (fl) {
-        rec.y = (uint32_t)jb.base;
-        for (int b2 = 0; b2 < jb.np; ++b2)
-          rec.y |= (uint32_t)jb.others[b2] << (8 * (1 + b2));
-        // suffix in right-fold order: last factor first
-        for (int b2 = 0; b2 < jb.ns; ++b2)
-          rec.z |= (uint32_t)jb.others[jb.np + jb.ns - 1 - b2] << (8 * b2);
+        rec = short_record(jb);
       } else {
         uint64_t offs = (uint64_t)jb.base;
         for (int b2 = 0; b2 < jb.np + jb.ns; ++b2)

@@ -56,6 +56,9 @@ static_assert(sizeof(WTerm) == 32, "staged copies move terms in 16-byte units");
 //  general: x = term | e << 20, or WJOB_NONE past the last job;
 //    (y, z) = offsets [pw(e-1, j), the other factors ascending];
 //    w = prefix count | suffix count << 4.
+// The 16-lane grid with the short layout deals whole sums to lanes instead
+// (run layout, jac_runs): x, y, z as above, w = V index of the step's four
+// term values | sum index << 16 | last step of the sum << 31.
 constexpr uint32_t WJOB_NONE = 0xFFFFFFFFu;
 
 // Request fields of the grid unit, packed in Req::gfold: largest number of
@@ -64,6 +67,8 @@ constexpr uint32_t WJOB_NONE = 0xFFFFFFFFu;
 PHC_D int gf_maxp(int g) { return (g >> 8) & 255; }
 PHC_D int gf_maxs(int g) { return (g >> 16) & 255; }
 PHC_D int gf_fl(int g) { return (g >> 24) & 15; }
+// Run layout (16-lane grid, short layout): see jac_runs.
+PHC_D bool gf_runs(int g) { return (g >> 28) & 1; }
 
 // LDS of one path. V holds the term values of equation e at
 // V[e * vstride ...], zero-padded to vstride, then vstride zeros that the
@@ -74,6 +79,13 @@ struct GridShape {
   static constexpr int R = LP / G;                       // column groups
   static constexpr int E = G / R;                        // columns per lane
 };
+
+// Job records in a request's table: [njobs][LP] in the run layout,
+// [njobs][LP][E] otherwise.
+template <int G>
+PHC_D int job_records(const Req& R) {
+  return R.njobs * GridShape<G>::LP * (gf_runs(R.gfold) ? 1 : GridShape<G>::E);
+}
 
 template <int G>
 struct WaveLds {
@@ -268,6 +280,52 @@ PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
   }
 }
 
+// Jacobian entries and values F_i of the 16-lane grid in the run layout:
+// each sum is a run of consecutive job steps on one lane, and the runs are
+// dealt over all 64 lanes, so no lane adds the jobs of several slots in
+// lock step while others idle. A lane adds its steps in order into one
+// running sum -- per step the job's derivative, then four term values, as
+// jac_short does -- and at a run's last step (record w bit 31) stores the
+// sum to S[w bits 16..30] and starts the next run from +0. Table
+// [m][LP], two steps per turn with the next turn's records prefetched.
+template <int FL>
+PHC_D void jac_runs(const Req& R, const WTerm* __restrict__ wt,
+                    const uint4* __restrict__ jobs, const cx* __restrict__ T,
+                    const cx* __restrict__ V, int q, cx* __restrict__ S) {
+  constexpr int LP = 64, U = 2;
+  const int M = R.njobs;
+  cx cur = mk(0.0, 0.0);
+  uint4 jn[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (u < M) jn[u] = jobs[(size_t)u * LP + q];
+  for (int m = 0; m < M; m += U) {
+    uint4 jc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jc[u] = jn[u];
+    if (m + U < M) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) jn[u] = jobs[(size_t)(m + U + u) * LP + q];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint4 job = jc[u];
+      const cx d = job_product<FL>(job, wt, T);
+      const int vb = (int)(job.w & 0xFFFFu);
+      const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
+      cur = cadd(cur, d);
+      cur = cadd(cur, v0);
+      cur = cadd(cur, v1);
+      cur = cadd(cur, v2);
+      cur = cadd(cur, v3);
+      if (job.w >> 31) {
+        S[(job.w >> 16) & 0x7FFFu] = cur;
+        cur = mk(0.0, 0.0);
+      }
+    }
+  }
+}
+
 // The same with the general job layout (any number of factors up to 8).
 template <int G>
 PHC_D void jac_general(const Req& R, const WTerm* __restrict__ wt,
@@ -417,14 +475,35 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     vcount[e] = fl[e] ? R.first[col[e] + 1] - R.first[col[e]] : 0;
   }
   cx acc[E];
-  if (fl_len == 1)
-    jac_short<G, 1>(R, wt, jobs, T, V, q, vbase, acc);
-  else if (fl_len == 2)
-    jac_short<G, 2>(R, wt, jobs, T, V, q, vbase, acc);
-  else if (fl_len == 3)
-    jac_short<G, 3>(R, wt, jobs, T, V, q, vbase, acc);
-  else
-    jac_general<G>(R, wt, jobs, T, V, q, vbase, vcount, acc);
+  if constexpr (E > 1) {
+    if (gf_runs(R.gfold)) {
+      // sums into S = W.A (free until the factors are stored), then each
+      // slot takes its own: entry (i, j) at i N + j, F_i at n N + i
+      if (fl_len == 1)
+        jac_runs<1>(R, wt, jobs, T, V, q, W.A);
+      else if (fl_len == 2)
+        jac_runs<2>(R, wt, jobs, T, V, q, W.A);
+      else
+        jac_runs<3>(R, wt, jobs, T, V, q, W.A);
+      wave_lds_fence();
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        acc[e] = jl[e]   ? W.A[lane * N + col[e]]
+                 : fl[e] ? W.A[n * N + col[e]]
+                         : mk(0.0, 0.0);
+    } else {
+      jac_general<G>(R, wt, jobs, T, V, q, vbase, vcount, acc);
+    }
+  } else {
+    if (fl_len == 1)
+      jac_short<G, 1>(R, wt, jobs, T, V, q, vbase, acc);
+    else if (fl_len == 2)
+      jac_short<G, 2>(R, wt, jobs, T, V, q, vbase, acc);
+    else if (fl_len == 3)
+      jac_short<G, 3>(R, wt, jobs, T, V, q, vbase, acc);
+    else
+      jac_general<G>(R, wt, jobs, T, V, q, vbase, vcount, acc);
+  }
 
   // 4. homotopy (src/homotopy.f90:491-511) and the patch row; the slot
   // that computes H_i and H_t,i keeps the right-hand side of row i in rhs:
