@@ -513,31 +513,36 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   {
     const cx gt = cmul(R.gamma, t);
     const cx omt = mk(1.0 - t.re, 0.0 - t.im);
+    const cx w = T[G + n];
+    // start-system terms of row `lane` (every Jacobian slot of a lane is in
+    // that row), formed once instead of per slot
+    const int il = lane < n ? lane : 0;
+    const int dl = R.deg[il];
+    const cx diag_add = cmul(cscale((double)dl, gt), zpowi(T[G + il], dl - 1));
+    const cx wcol_sub =
+        cmul(cscale((double)dl, cmul(gt, R.b[il])), zpowi(w, dl - 1));
     static_for<E>([&](auto ec) {
       constexpr int e = decltype(ec)::value;
       const int j = col[e];
       a[e] = mk(0.0, 0.0);
       rhs[e] = mk(0.0, 0.0);
-      if (jl[e] || fl[e]) {
-        const int i = jl[e] ? lane : j;
+      if (jl[e]) {
+        a[e] = cmul(omt, acc[e]);
+        if (j == lane) a[e] = cadd(a[e], diag_add);
+        if (j == n) a[e] = csub(a[e], wcol_sub);
+      } else if (fl[e]) {
+        const int i = j;
         const int d = R.deg[i];
         const cx zi = T[G + i];
-        const cx w = T[G + n];
         const cx zd1 = zpowi(zi, d - 1);
         const cx wd1 = zpowi(w, d - 1);
         const cx bi = R.b[i];
-        if (jl[e]) {
-          a[e] = cmul(omt, acc[e]);
-          if (j == lane) a[e] = cadd(a[e], cmul(cscale((double)d, gt), zd1));
-          if (j == n) a[e] = csub(a[e], cmul(cscale((double)d, cmul(gt, bi)), wd1));
-        } else {
-          const cx g = csub(cmul(zd1, zi), cmul(cmul(bi, wd1), w));
-          const cx h = cadd(cmul(gt, g), cmul(omt, acc[e]));
-          const cx ht = csub(cmul(R.gamma, g), acc[e]);
-          W.H[i] = h;
-          W.HT[i] = ht;
-          rhs[e] = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
-        }
+        const cx g = csub(cmul(zd1, zi), cmul(cmul(bi, wd1), w));
+        const cx h = cadd(cmul(gt, g), cmul(omt, acc[e]));
+        const cx ht = csub(cmul(R.gamma, g), acc[e]);
+        W.H[i] = h;
+        W.HT[i] = ht;
+        rhs[e] = (mode == PC_TAN) ? cneg(cmul(ht, dt)) : cneg(h);
       }
       if (lane == n && j < N) a[e] = R.patch[j];
       if (lane == n && j == n) {
