@@ -352,14 +352,18 @@ bool build_grid(HostReq& h, int G) {
   size_t M = 0;
   for (const auto& l : lists) M = std::max(M, l.size());
   M = std::max(M, (size_t)((maxterms + 3) / 4));
-  if (fl && E == 1) M += M & 1;  // short layout: two rows per step
+  if (E == 1) M += M & 1;  // one slot per lane: two rows per step
   const int vstride = (int)(4 * M);
   for (int i = 0; i < n; ++i)
     for (int k = r.first[i]; k < r.first[i + 1]; ++k)
       wt[k].slot = (uint32_t)(i * vstride + (k - r.first[i]));
   wt[nt].slot = (uint32_t)(n * vstride);  // never written: not a real term
-  const uint4 pad = fl ? make_uint4((uint32_t)nt | (1u << 20), 0u, 0u, 0u)
-                       : make_uint4(WJOB_NONE, 0u, 0u, 0u);
+  // padding rows: the zero-coefficient term with no other factors (adds
+  // +0), except for the general layout of the 16-lane grid, whose loop
+  // skips WJOB_NONE rows
+  const uint4 pad = (fl || E == 1)
+                        ? make_uint4((uint32_t)nt | (1u << 20), 0u, 0u, 0u)
+                        : make_uint4(WJOB_NONE, 0u, 0u, 0u);
   h.jobs.assign(M * LP * E, pad);
   for (int qq = 0; qq < LP * E; ++qq) {
     for (size_t m = 0; m < lists[qq].size(); ++m) {
