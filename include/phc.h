@@ -254,9 +254,11 @@ int phc_ctx_autotune(phc_ctx *ctx, const phc_system *sys,
 int phc_ctx_clear_tuning(phc_ctx *ctx);
 
 /* ---- batched multi-system tracker ------------------------------------- */
-/* Tracks the paths of nsys independent solves in one persistent launch
- * sequence; system s has npaths[s] paths with start points start[s] and
- * writes out[s]. Equivalent, path by path, to nsys phc_track_paths calls. */
+/* Tracks the paths of nsys independent solves: one persistent launch per
+ * lanes class (4, 8, 16), the classes running concurrently on their own
+ * streams; system s has npaths[s] paths with start points start[s] and
+ * writes out[s]. Equivalent, path by path, to nsys phc_track_paths calls.
+ * Synchronous. */
 int phc_track_batch(phc_ctx *ctx, int32_t nsys, const phc_system *sys,
                     const phc_homotopy *hom, const phc_solve_options *opts,
                     const int64_t *npaths, const double *const *start,
