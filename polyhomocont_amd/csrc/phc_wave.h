@@ -597,9 +597,20 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     // that row), formed once instead of per slot
     const int il = lane < n ? lane : 0;
     const int dl = R.deg[il];
-    const cx diag_add = cmul(cscale((double)dl, gt), zpowi(T[G + il], dl - 1));
-    const cx wcol_sub =
-        cmul(cscale((double)dl, cmul(gt, R.b[il])), zpowi(w, dl - 1));
+    const cx zl = T[G + il];
+    const cx zdl = zpowi(zl, dl - 1);
+    const cx wdl = zpowi(w, dl - 1);
+    const cx bl = R.b[il];
+    const cx diag_add = cmul(cscale((double)dl, gt), zdl);
+    const cx wcol_sub = cmul(cscale((double)dl, cmul(gt, bl)), wdl);
+    // with several slots per lane, the F slots of a lane would each form
+    // their equation's start-system value; group 0's row lanes form it once
+    // and pass it through HT, which the F slots overwrite after reading
+    if constexpr (E > 1) {
+      if (rep == 0 && lane < n)
+        W.HT[lane] = csub(cmul(zdl, zl), cmul(cmul(bl, wdl), w));
+      wave_lds_fence();
+    }
     static_for<E>([&](auto ec) {
       constexpr int e = decltype(ec)::value;
       const int j = col[e];
@@ -611,12 +622,17 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
         if (j == n) a[e] = csub(a[e], wcol_sub);
       } else if (fl[e]) {
         const int i = j;
-        const int d = R.deg[i];
-        const cx zi = T[G + i];
-        const cx zd1 = zpowi(zi, d - 1);
-        const cx wd1 = zpowi(w, d - 1);
-        const cx bi = R.b[i];
-        const cx g = csub(cmul(zd1, zi), cmul(cmul(bi, wd1), w));
+        cx g;
+        if constexpr (E > 1) {
+          g = W.HT[i];
+        } else {
+          const int d = R.deg[i];
+          const cx zi = T[G + i];
+          const cx zd1 = zpowi(zi, d - 1);
+          const cx wd1 = zpowi(w, d - 1);
+          const cx bi = R.b[i];
+          g = csub(cmul(zd1, zi), cmul(cmul(bi, wd1), w));
+        }
         const cx h = cadd(cmul(gt, g), cmul(omt, acc[e]));
         const cx ht = csub(cmul(R.gamma, g), acc[e]);
         W.H[i] = h;
