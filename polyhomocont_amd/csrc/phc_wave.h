@@ -497,6 +497,68 @@ PHC_D void pivot_search(bool cand, cx ak, int pos, int lane, int src,
   bl = pbcast_i<LP>(bl, src);
 }
 
+// Forward and back substitution (src/linalg.f90:93-111) for NC rows known
+// at compile time, row per lane: lane i of every group holds row i of the
+// factors and runs the recurrence of that row; the value of the row at
+// logical position k is broadcast from its lane (lane PL[k] of the group,
+// PL packed in plb). Sums in the reference's order, as in the
+// row-per-lane unit. With NC a constant the steps carry no row-count
+// tests (such tests inside the unrolled steps cost as much as the steps).
+template <int G, int NC>
+PHC_D cx solve_rows(const cx (&arow)[G], const cx bown, const cdiv_den pdiag,
+                    int pos, bool row, int lane, uint64_t plb) {
+  constexpr int LP = GridShape<G>::LP;
+  auto rowcast = [&](cx v, int r) -> cx {
+    if (LP == 64) return pbcast_c<LP>(v, r);  // group 0, lane r
+    return gshfl_c<G>(v, r);
+  };
+  cx sacc = mk(0.0, 0.0);
+  bool sset = false;
+  static_for<NC>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+    const cx yk = rowcast(cand, (int)((plb >> (4 * k)) & 15u));
+    if (row && pos > k) {
+      const cx p = cmul(arow[k], yk);
+      sacc = sset ? cadd(sacc, p) : p;
+      sset = true;
+    }
+  });
+  const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+  cx xs[NC];
+  cx xmine = mk(0.0, 0.0);
+  static_for<NC>([&](auto ic) {
+    constexpr int i = NC - 1 - decltype(ic)::value;
+    cx num = yfin;
+    if constexpr (i + 1 < NC) {
+      cx sm = cmul(arow[i + 1], xs[i + 1]);
+      static_for<NC>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if constexpr (j >= i + 2) sm = cadd(sm, cmul(arow[j], xs[j]));
+      });
+      num = csub(yfin, sm);
+    }
+    const cx cand = cdiv_apply(num, pdiag);
+    xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
+    if (lane == i) xmine = xs[i];
+  });
+  return xmine;
+}
+
+// solve_rows for the path's N among NC .. G (a lanes class of G serves
+// N from G / 2 + 1 to G; the 4-lane class from 2).
+template <int G, int NC>
+PHC_D cx solve_dispatch(int N, const cx (&arow)[G], const cx bown,
+                        const cdiv_den pdiag, int pos, bool row, int lane,
+                        uint64_t plb) {
+  if constexpr (NC < G) {
+    if (N != NC)
+      return solve_dispatch<G, NC + 1>(N, arow, bown, pdiag, pos, row, lane,
+                                       plb);
+  }
+  return solve_rows<G, NC>(arow, bown, pdiag, pos, row, lane, plb);
+}
+
 // Factors that are exactly one: the reference multiplies by pw(0, j) = 1
 // for every zero exponent (src/sparse.f90:262, :266); this unit multiplies
 // by 1 only to pad a lane's product to the length shared by the
@@ -765,11 +827,9 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     return;
   }
 
-  // 6. forward and back substitution (src/linalg.f90:93-111), row per
-  // lane: lane i of every group holds row i of the factors and runs the
-  // recurrence of that row; the value of the row at logical position k is
-  // broadcast from its lane (lane PL[k] of the group). Sums in the
-  // reference's order, as in the row-per-lane unit.
+  // 6. forward and back substitution (src/linalg.f90:93-111): solve_rows,
+  // for this path's N (the host gives a lanes class of G only paths with
+  // G / 2 < N <= G, or 2 <= N <= 4 for G = 4)
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     if (in[e]) W.A[lane * G + col[e]] = a[e];
@@ -793,80 +853,8 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     const int orow = lane < n ? G - 1 : n;          // its row
     bown = W.RB[((oc % RG) * G + orow) * E + oc / RG];
   }
-  auto rowcast = [&](cx v, int r) -> cx {
-    if (LP == 64) return pbcast_c<LP>(v, r);  // group 0, lane r
-    return gshfl_c<G>(v, r);
-  };
-  if constexpr (E == 1) {
-    cx sacc = mk(0.0, 0.0);
-    bool sset = false;
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-      if (k >= N) break;
-      const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
-      const cx yk = rowcast(cand, (int)((plb >> (4 * k)) & 15u));
-      if (row && pos > k) {
-        const cx p = cmul(arow[k], yk);
-        sacc = sset ? cadd(sacc, p) : p;
-        sset = true;
-      }
-    }
-    const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
-    cx xs[G];
-    cx xmine = mk(0.0, 0.0);
-#pragma unroll
-    for (int i = G - 1; i >= 0; --i) {
-      if (i < N) {
-        cx num = yfin;
-        if (i + 1 < N) {
-          cx sm = cmul(arow[i + 1], xs[i + 1]);
-#pragma unroll
-          for (int j = i + 2; j < G; ++j)
-            if (j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
-          num = csub(yfin, sm);
-        }
-        const cx cand = cdiv_apply(num, pdiag);
-        xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
-        if (lane == i) xmine = xs[i];
-      }
-    }
-    xout = xmine;
-  } else {
-    // arow and xs are indexed by compile-time constants only (static_for)
-    cx sacc = mk(0.0, 0.0);
-    bool sset = false;
-    static_for<G>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      if (k >= N) return;
-      const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
-      const cx yk = rowcast(cand, (int)((plb >> (4 * k)) & 15u));
-      if (row && pos > k) {
-        const cx p = cmul(arow[k], yk);
-        sacc = sset ? cadd(sacc, p) : p;
-        sset = true;
-      }
-    });
-    const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
-    cx xs[G];
-    cx xmine = mk(0.0, 0.0);
-    static_for<G>([&](auto ic) {
-      constexpr int i = G - 1 - decltype(ic)::value;
-      if (i >= N) return;
-      cx num = yfin;
-      if (i + 1 < N) {
-        cx sm = cmul(arow[(i + 1) % G], xs[(i + 1) % G]);
-        static_for<G>([&](auto jc) {
-          constexpr int j = decltype(jc)::value;
-          if (j >= i + 2 && j < N) sm = cadd(sm, cmul(arow[j], xs[j]));
-        });
-        num = csub(yfin, sm);
-      }
-      const cx cand = cdiv_apply(num, pdiag);
-      xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
-      if (lane == i) xmine = xs[i];
-    });
-    xout = xmine;
-  }
+  xout = solve_dispatch<G, (G == 4 ? 2 : G / 2 + 1)>(N, arow, bown, pdiag,
+                                                      pos, row, lane, plb);
 }
 
 }  // namespace phc
