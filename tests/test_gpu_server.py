@@ -31,6 +31,9 @@ def mixed_requests(host):
              for n, d in ((2, 3), (3, 2), (5, 2), (4, 3))]
     reqs += [request(host, W.config_system("c1")),
              request(host, W.ref_test_systems()["mixed"], seed=4)]
+    # 16-lane class (n = 8): the run layout of the grid unit
+    reqs += [request(host, W.random_sparse(104 + 66), limit=200),
+             request(host, W.random_sparse(104 + 68), seed=2, limit=150)]
     return reqs
 
 
