@@ -497,6 +497,59 @@ PHC_D void pivot_search(bool cand, cx ak, int pos, int lane, int src,
   bl = pbcast_i<LP>(bl, src);
 }
 
+// LU steps of a grid with several slots per lane (src/linalg.f90:49-76).
+// Step k works on slot ke = k / RG of group kr = k % RG; the steps of one
+// slot share a compile-time ke, so the slot arrays stay in registers.
+// EU = ceil(N / RG) slots hold columns of the system; it is a template
+// argument so that the slot loops carry no runtime bound. Returns 0 on a
+// zero pivot.
+template <int G, int EU>
+PHC_D int lu_slots(int N, cx (&a)[GridShape<G>::E],
+                   const bool (&in)[GridShape<G>::E],
+                   const int (&col)[GridShape<G>::E], int rep, int lane,
+                   int& pos, uint64_t& plb) {
+  constexpr int LP = GridShape<G>::LP, RG = GridShape<G>::R,
+                E = GridShape<G>::E;
+  int ok = 1;
+  static_for<E>([&](auto ec) {
+    constexpr int ke = decltype(ec)::value;
+    if constexpr (ke < EU) {
+#pragma unroll
+      for (int kr = 0; kr < RG; ++kr) {
+        const int k = ke * RG + kr;
+        if (!ok || k >= N) break;
+        const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
+        double bv;
+        int bp, bl;
+        pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, lane,
+                            kr * G, bv, bp, bl);
+        if (bv == 0.0) {
+          ok = 0;
+          break;
+        }
+        plb |= (uint64_t)bl << (4 * k);
+        const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
+        // slots left of ke hold finished columns (col[e] < k)
+        cx pr[E];
+#pragma unroll
+        for (int e = ke; e < EU; ++e) pr[e] = pshfl_c<LP>(a[e], rep * G + bl);
+        int np = pos;
+        if (pos == k) np = bp;
+        if (lane == bl) np = k;
+        pos = np;
+        const cx li = cdiv(aik, piv);
+        if (in[ke] && rep == kr && pos > k) a[ke] = li;
+#pragma unroll
+        for (int e = ke; e < EU; ++e) {
+          const cx upd = csub(a[e], cmul(li, pr[e]));
+          if (in[e] && col[e] > k && pos > k) a[e] = upd;
+        }
+      }
+    }
+  });
+  return ok;
+}
+
 // Forward and back substitution (src/linalg.f90:93-111) for NC rows known
 // at compile time, row per lane: lane i of every group holds row i of the
 // factors and runs the recurrence of that row; the value of the row at
@@ -778,48 +831,12 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
       }
     }
   } else {
-    // Step k works on slot ke = k / RG of group kr = k % RG; the steps of
-    // one slot share a compile-time ke, so the slot arrays stay in registers
-    // whichever way the step loop is compiled (the 16-lane unit's is not
-    // unrolled).
+    // the slot count eu = ceil(N / RG) takes two values per class
     const int eu = slots_used<G>(N);
-    static_for<E>([&](auto ec) {
-      constexpr int ke = decltype(ec)::value;
-#pragma unroll
-      for (int kr = 0; kr < RG; ++kr) {
-        const int k = ke * RG + kr;
-        if (!ok || k >= N) break;
-        const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
-        double bv;
-        int bp, bl;
-        pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, lane,
-                            kr * G, bv, bp, bl);
-        if (bv == 0.0) {
-          ok = 0;
-          break;
-        }
-        plb |= (uint64_t)bl << (4 * k);
-        const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
-        // slots left of ke hold finished columns (col[e] < k), slots from
-        // eu on hold no column of the system
-        cx pr[E];
-#pragma unroll
-        for (int e = ke; e < E; ++e)
-          if (e < eu) pr[e] = pshfl_c<LP>(a[e], rep * G + bl);
-        int np = pos;
-        if (pos == k) np = bp;
-        if (lane == bl) np = k;
-        pos = np;
-        const cx li = cdiv(aik, piv);
-        if (in[ke] && rep == kr && pos > k) a[ke] = li;
-#pragma unroll
-        for (int e = ke; e < E; ++e) {
-          if (e >= eu) break;
-          const cx upd = csub(a[e], cmul(li, pr[e]));
-          if (in[e] && col[e] > k && pos > k) a[e] = upd;
-        }
-      }
-    });
+    if (eu == E)
+      ok = lu_slots<G, E>(N, a, in, col, rep, lane, pos, plb);
+    else
+      ok = lu_slots<G, (E > 1 ? E - 1 : 1)>(N, a, in, col, rep, lane, pos, plb);
   }
   okout = ok;
   if (!ok) {
