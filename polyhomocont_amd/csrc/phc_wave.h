@@ -218,65 +218,42 @@ PHC_D cx job_product(const uint4 job, const WTerm* __restrict__ wt,
   return cmul(cmul(d, P), S);
 }
 
-// Jacobian entries of the lane's E slots (src/sparse.f90:271-276) and, in
+// Jacobian entry of the lane's one slot (src/sparse.f90:271-276) or, in
 // the value slots, F_e = sum of its equation's values in term order, four
-// per job. Short job layout, no branches; U jobs per step (the job rows are
-// padded to a multiple of U). Job table [m][LP][E].
+// per job. Short job layout, no branches; two jobs per turn (the job rows
+// are padded to an even count), the next turn's records loaded while this
+// one computes. Job table [m][LP]. Grids with several slots per lane use
+// the run layout (jac_runs) instead.
 template <int G, int FL>
 PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
                      const uint4* __restrict__ jobs, const cx* __restrict__ T,
-                     const cx* __restrict__ V, int q, const int (&vbase)[GridShape<G>::E],
-                     cx (&acc)[GridShape<G>::E]) {
-  constexpr int LP = GridShape<G>::LP, E = GridShape<G>::E;
-  constexpr int U = E == 1 ? 2 : 1;
+                     const cx* __restrict__ V, int q, int vbase, cx& acc) {
+  static_assert(GridShape<G>::E == 1, "one slot per lane");
+  constexpr int LP = GridShape<G>::LP, U = 2;
   const int M = R.njobs;
-  const int eu = slots_used<G>(R.N);
+  acc = mk(0.0, 0.0);
+  uint4 jn[U];
 #pragma unroll
-  for (int e = 0; e < E; ++e) acc[e] = mk(0.0, 0.0);
-  // one slot per lane: the next step's job records are loaded while this
-  // step computes (with four slots the extra registers cost more than the
-  // latency they hide)
-  constexpr bool PREFETCH = E == 1;
-  uint4 jn[U][E];
-  if (PREFETCH) {
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (u < M) jn[u][0] = jobs[(size_t)u * LP + q];
-  }
+  for (int u = 0; u < U; ++u)
+    if (u < M) jn[u] = jobs[(size_t)u * LP + q];
   for (int m = 0; m < M; m += U) {
-    uint4 jc[U][E];
+    uint4 jc[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int u = 0; u < U; ++u) jc[u] = jn[u];
+    if (m + U < M) {
 #pragma unroll
-      for (int e = 0; e < E; ++e)
-        if (PREFETCH)
-          jc[u][e] = jn[u][e];
-        else if (e < eu)
-          jc[u][e] = jobs[((size_t)(m + u) * LP + q) * E + e];
-    if (PREFETCH && m + U < M) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) jn[u][0] = jobs[(size_t)(m + U + u) * LP + q];
+      for (int u = 0; u < U; ++u) jn[u] = jobs[(size_t)(m + U + u) * LP + q];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      uint4 job[E];
-#pragma unroll
-      for (int e = 0; e < E; ++e) job[e] = jc[u][e];
-      cx d[E];
-#pragma unroll
-      for (int e = 0; e < E; ++e)
-        if (e < eu) d[e] = job_product<FL>(job[e], wt, T);
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        if (e >= eu) continue;
-        const int vb = vbase[e] + 4 * (m + u);
-        const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
-        acc[e] = cadd(acc[e], d[e]);
-        acc[e] = cadd(acc[e], v0);
-        acc[e] = cadd(acc[e], v1);
-        acc[e] = cadd(acc[e], v2);
-        acc[e] = cadd(acc[e], v3);
-      }
+      const cx d = job_product<FL>(jc[u], wt, T);
+      const int vb = vbase + 4 * (m + u);
+      const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
+      acc = cadd(acc, d);
+      acc = cadd(acc, v0);
+      acc = cadd(acc, v1);
+      acc = cadd(acc, v2);
+      acc = cadd(acc, v3);
     }
   }
 }
@@ -690,11 +667,11 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     }
   } else {
     if (fl_len == 1)
-      jac_short<G, 1>(R, wt, jobs, T, V, q, vbase, acc);
+      jac_short<G, 1>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
     else if (fl_len == 2)
-      jac_short<G, 2>(R, wt, jobs, T, V, q, vbase, acc);
+      jac_short<G, 2>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
     else if (fl_len == 3)
-      jac_short<G, 3>(R, wt, jobs, T, V, q, vbase, acc);
+      jac_short<G, 3>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
     else
       jac_general1<G>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
   }
