@@ -252,6 +252,35 @@ def test_wide_systems_bit_exact(ctx, ref, mapping):
         ctx.set_mapping(0)
 
 
+def _wide_many_factor_system(n=8):
+    """n = 8 (16-lane class) with a five-variable term in every equation:
+    more than four nonzero exponents per term, so the grid unit takes its
+    general job layout."""
+    rng = np.random.default_rng(77)
+    eye = np.eye(n, dtype=np.int32)
+    eqs = []
+    for i in range(n):
+        five = np.zeros(n, np.int32)
+        five[[(i + k) % n for k in range(5)]] = 1
+        e = np.vstack([five[None, :], eye, np.zeros((1, n), np.int32)])
+        c = rng.normal(size=len(e)) + 1j * rng.normal(size=len(e))
+        eqs.append((c, e))
+    return abi.UserSystemData(n, eqs)
+
+
+def test_wide_general_layout_bit_exact(ctx, ref):
+    """The 16-lane grid with terms of five factors (general job layout)."""
+    usys = _wide_many_factor_system()
+    opts = abi.default_options(seed=5)
+    g, b, a, ns = ref.homotopy(usys, opts.seed)
+    first, cnt = 1000, 6
+    start = ref.start_points(usys, opts.seed, first, cnt)
+    print(f"[gpu] wide general layout: paths {first}..{first + cnt} of {ns}",
+          flush=True)
+    got = run_device(ctx, usys, opts, (g, b, a), start)
+    assert_bit_exact(got, ref.endpoints(usys, opts, first, cnt), cnt)
+
+
 def test_autotuned_launch_is_used_and_bit_exact(ctx, host):
     """The autotuner's choice is applied to later launches of the same
     shape, and the records stay the reference's."""
