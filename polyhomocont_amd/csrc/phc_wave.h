@@ -304,53 +304,45 @@ PHC_D void jac_runs(const Req& R, const WTerm* __restrict__ wt,
   }
 }
 
-// One derivative of a general-layout job record: prefix and suffix
-// products padded with pw(0, 0) = 1 to maxp and maxs factors, as in
-// jac_general.
+// One derivative of a general-layout job record, prefix and suffix
+// products padded with pw(0, 0) = 1 to FLG factors each (the short layout
+// pads the same way); FLG = max(maxp, maxs) is a template argument, so the
+// factor loops carry no runtime bound.
+template <int FLG>
 PHC_D cx job_product_general(const uint4 job, const WTerm* __restrict__ wt,
-                             const cx* __restrict__ T, int maxp, int maxs) {
-  constexpr int FM = 8;
+                             const cx* __restrict__ T) {
   const int ex = (int)((job.x >> 20) & 255u);
   const uint64_t off = ((uint64_t)job.z << 32) | job.y;
   const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
   const WTerm& w = wt[job.x & 0xFFFFFu];
   const cx c = mk(w.re, w.im);
   const cx B = T[woff(off, 0)];
-  cx Pf[FM], Sf[FM];
+  cx Pf[FLG], Sf[FLG];
 #pragma unroll
-  for (int b = 0; b < FM; ++b) {
-    if (b < maxp) Pf[b] = T[b < np ? woff(off, 1 + b) : 0];
-    if (b < maxs) Sf[b] = T[b < ns ? woff(off, np + ns - b) : 0];
+  for (int b = 0; b < FLG; ++b) {
+    Pf[b] = T[b < np ? woff(off, 1 + b) : 0];
+    Sf[b] = T[b < ns ? woff(off, np + ns - b) : 0];
   }
   cx d = cmul(cscale((double)ex, c), B);
-  if (maxp > 0) {
-    cx P = Pf[0];
+  cx P = Pf[0], S = Sf[0];
 #pragma unroll
-    for (int b = 1; b < FM; ++b)
-      if (b < maxp) P = cmul(P, Pf[b]);
-    d = cmul(d, P);
+  for (int b = 1; b < FLG; ++b) {
+    P = cmul(P, Pf[b]);
+    S = cmul(Sf[b], S);
   }
-  if (maxs > 0) {
-    cx S = Sf[0];
-#pragma unroll
-    for (int b = 1; b < FM; ++b)
-      if (b < maxs) S = cmul(Sf[b], S);
-    d = cmul(d, S);
-  }
-  return d;
+  return cmul(cmul(d, P), S);
 }
 
 // General job layout with one slot per lane: the loop of jac_short (two
 // jobs per turn, next turn's records prefetched, four term values per
 // step) around job_product_general. Rows past a lane's last job are the
 // zero-coefficient term with no other factors, so they add +0.
-template <int G>
+template <int G, int FLG>
 PHC_D void jac_general1(const Req& R, const WTerm* __restrict__ wt,
                         const uint4* __restrict__ jobs,
                         const cx* __restrict__ T, const cx* __restrict__ V,
                         int q, int vbase, cx& acc) {
   constexpr int LP = GridShape<G>::LP, U = 2;
-  const int maxp = gf_maxp(R.gfold), maxs = gf_maxs(R.gfold);
   const int M = R.njobs;
   acc = mk(0.0, 0.0);
   uint4 jn[U];
@@ -365,21 +357,36 @@ PHC_D void jac_general1(const Req& R, const WTerm* __restrict__ wt,
 #pragma unroll
       for (int u = 0; u < U; ++u) jn[u] = jobs[(size_t)(m + U + u) * LP + q];
     }
-    cx d[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      d[u] = job_product_general(jc[u], wt, T, maxp, maxs);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      const cx d = job_product_general<FLG>(jc[u], wt, T);
       const int vb = vbase + 4 * (m + u);
       const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
-      acc = cadd(acc, d[u]);
+      acc = cadd(acc, d);
       acc = cadd(acc, v0);
       acc = cadd(acc, v1);
       acc = cadd(acc, v2);
       acc = cadd(acc, v3);
     }
   }
+}
+
+// jac_general1 for FLG = max(maxp, maxs) among NC .. 7 (a term has at
+// most 8 nonzero exponents, so a job has at most 7 other factors).
+template <int G, int NC>
+PHC_D void jac_general1_dispatch(int flg, const Req& R,
+                                 const WTerm* __restrict__ wt,
+                                 const uint4* __restrict__ jobs,
+                                 const cx* __restrict__ T,
+                                 const cx* __restrict__ V, int q, int vbase,
+                                 cx& acc) {
+  if constexpr (NC < 7) {
+    if (flg > NC) {
+      jac_general1_dispatch<G, NC + 1>(flg, R, wt, jobs, T, V, q, vbase, acc);
+      return;
+    }
+  }
+  jac_general1<G, NC>(R, wt, jobs, T, V, q, vbase, acc);
 }
 
 // The same with the general job layout (any number of factors up to 8).
@@ -673,7 +680,10 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
     else if (fl_len == 3)
       jac_short<G, 3>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
     else
-      jac_general1<G>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
+      jac_general1_dispatch<G, 1>(
+          gf_maxp(R.gfold) > gf_maxs(R.gfold) ? gf_maxp(R.gfold)
+                                              : gf_maxs(R.gfold),
+          R, wt, jobs, T, V, q, vbase[0], acc[0]);
   }
 
   // 4. homotopy (src/homotopy.f90:491-511) and the patch row; the slot
