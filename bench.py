@@ -307,8 +307,8 @@ def main():
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; the path "
               "tracker runs on one GPU (paths would shard by index range "
-              "across GPUs with a host gather, see DESIGN.md)",
-              file=sys.stderr)
+              "across GPUs, with the records gathered to one host in path "
+              "order for classification)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
