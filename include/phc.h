@@ -55,7 +55,11 @@ extern "C" {
 #define PHC_ERR_SINGULAR_MATRIX 2
 #define PHC_ERR_NUMERICAL 3
 #define PHC_ERR_DEVICE (-1)      /* no usable GPU, or a HIP call failed      */
-#define PHC_ERR_CAPACITY (-2)    /* request exceeds a configured limit       */
+#define PHC_ERR_CAPACITY (-2)    /* request exceeds a configured limit, or a
+                                    valid system the kernels cannot hold
+                                    (N > PHC_MAX_UNKNOWNS, an exponent above
+                                    PHC_MAX_EXPONENT): the Fortran `solve`
+                                    tracks such systems on the host         */
 #define PHC_ERR_PENDING (-3)     /* server ticket not finished yet           */
 
 /* Per-path status (src/tracker.f90:45-51, src/solve.f90:53). */
@@ -69,8 +73,10 @@ extern "C" {
 #define PHC_ENDGAME_NOT_CONVERGED 1
 #define PHC_ENDGAME_FAILED 2
 
-/* Largest number of homotopy unknowns N = n + 1 the kernels support. */
+/* Largest number of homotopy unknowns N = n + 1, and largest exponent of a
+ * homogenized term, that the kernels support (term exponents are bytes). */
 #define PHC_MAX_UNKNOWNS 16
+#define PHC_MAX_EXPONENT 255
 
 /* ---- options ----------------------------------------------------------- */
 /* Field-for-field mirror of the reference option types; the defaults are the
@@ -200,8 +206,30 @@ int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
  * mapping. */
 int phc_ctx_set_mapping(phc_ctx *ctx, int mapping, int wg_threads);
 
-/* Last error message of the context (static storage, never NULL). */
+/* Last error message of the context (static storage, never NULL). With
+ * several threads on one context it may be another thread's message;
+ * phc_thread_last_error gives the calling thread's own. */
 const char *phc_ctx_last_error(const phc_ctx *ctx);
+/* Message of the last failed library call made by the calling thread
+ * (thread-local storage, never NULL; "" if none failed). */
+const char *phc_thread_last_error(void);
+
+/* Whether the kernels can track this system with these options, without
+ * touching a device: PHC_OK, PHC_ERR_INVALID_INPUT (the system or options
+ * are malformed) or PHC_ERR_CAPACITY (valid, but beyond the kernels'
+ * limits). The tracking entry points make the same checks. */
+int phc_check_system(const phc_system *sys, const phc_solve_options *opts);
+
+/* Process-wide context of a device, created on first use under a lock and
+ * kept until the process ends (one per device index): the device-context
+ * cache of callers that hold no context of their own, such as the Fortran
+ * `solve`. Never pass it to phc_ctx_destroy. PHC_ERR_DEVICE if the device
+ * cannot be used. */
+int phc_ctx_shared(int device, phc_ctx **ctx);
+/* Device index that callers without a context of their own use (default
+ * 0; -1 asks them to track on the host CPU). Atomic. */
+void phc_set_default_device(int device);
+int phc_get_default_device(void);
 
 /* ---- plain per-solve tracker ------------------------------------------ */
 /* Tracks paths first_path .. first_path+npaths-1 of one system (path k uses
@@ -288,6 +316,13 @@ int phc_server_progress(phc_server *srv);
  * not yet; phc_server_wait blocks (driving progress) until done. */
 int phc_server_poll(phc_server *srv, uint64_t ticket);
 int phc_server_wait(phc_server *srv, uint64_t ticket);
+/* Forgets a finished ticket (its records stay in the caller's `out`), or
+ * cancels one still queued for admission. PHC_ERR_PENDING while its paths
+ * are on the device; PHC_ERR_INVALID_INPUT for an unknown ticket. A
+ * long-running server calls it once per request, after poll/wait. */
+int phc_server_release(phc_server *srv, uint64_t ticket);
+/* counts[0] queued, [1] admitted, [2] finished and not yet released. */
+int phc_server_tickets(phc_server *srv, int64_t *counts);
 int phc_server_destroy(phc_server *srv);
 
 #ifdef __cplusplus

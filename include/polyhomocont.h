@@ -71,9 +71,25 @@ typedef struct phc_path_results {
 
 /* Runs `solve` on the system. Returns the `status` argument of solve
  * (0 or err_invalid_input); a negative value means the GPU path could not
- * run (the library does not fall back to the CPU for sparse systems). */
+ * run (err_device: solve does not fall back to the host when the device
+ * fails; phc_f_set_device(-1) asks for the host tracker explicitly).
+ * Systems the kernels cannot hold (more than 15 variables, an exponent
+ * above 255) are tracked on the host with the same results. */
 int phc_f_solve(const phc_user_system *sys, const phc_solve_options *opts,
                 phc_solve_summary *summary, phc_path_results *paths);
+
+/* `solve` on the same system wrapped as a user evaluator (a poly_system
+ * extension that supplies only the affine F(x) and its Jacobian, like the
+ * user types of the reference's tests): the path that the Fortran `solve`
+ * takes for every non-sparse system -- the host tracker, paths over the
+ * OpenMP threads with one copy of the system per thread. */
+int phc_f_solve_user(const phc_user_system *sys, const phc_solve_options *opts,
+                     phc_solve_summary *summary, phc_path_results *paths);
+
+/* Host threads of later solve calls from the calling thread (OpenMP's
+ * omp_set_num_threads; nthreads <= 0 leaves it unchanged). Returns the
+ * previous count. Results never depend on it. */
+int32_t phc_f_set_threads(int32_t nthreads);
 
 /* The random constants and start points `solve` uses for this system and
  * seed (tdh_init, tdh_start_solution). `nstart` receives the Bezout number.
@@ -95,7 +111,8 @@ int phc_f_classify(const phc_user_system *sys, const phc_solve_options *opts,
                    int64_t npaths, const phc_endpoints *rec,
                    phc_solve_summary *summary, phc_path_results *paths);
 
-/* Index of the HIP device used by phc_f_solve (default 0). */
+/* Index of the HIP device used by phc_f_solve (default 0; -1: track
+ * sparse systems on the host CPU). */
 void phc_f_set_device(int32_t device);
 
 #ifdef __cplusplus

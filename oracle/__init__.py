@@ -42,10 +42,18 @@ class _Oracle:
             "lu": ([i32, P, P, P], C.c_int),
             "cond1": ([i32, P], C.c_double),
         }
+        # entry points only the compiled reference's driver has
+        optional = {"solve_user": ([P, P, P, P], C.c_int)}
         for name, (args, res) in sig.items():
             f = getattr(self.lib, prefix + name)
             f.argtypes = args
             f.restype = res
+            setattr(self, "_" + name, f)
+        for name, (args, res) in optional.items():
+            f = getattr(self.lib, prefix + name, None)
+            if f is not None:
+                f.argtypes = args
+                f.restype = res
             setattr(self, "_" + name, f)
 
     # -- thin wrappers ------------------------------------------------------
@@ -84,6 +92,15 @@ class _Oracle:
         res = abi.SolveArrays(max(usys.bezout, 1), usys.n)
         st = self._solve(C.byref(usys.c), C.byref(opts), C.byref(res.summary),
                          C.byref(res.paths))
+        return st, res
+
+    def solve_user(self, usys, opts):
+        """solve on the system wrapped as a user evaluator (affine F only)."""
+        if self._solve_user is None:
+            raise NotImplementedError("only the compiled reference has it")
+        res = abi.SolveArrays(max(usys.bezout, 1), usys.n)
+        st = self._solve_user(C.byref(usys.c), C.byref(opts),
+                              C.byref(res.summary), C.byref(res.paths))
         return st, res
 
     def eval(self, usys, seed, z, t):

@@ -27,6 +27,7 @@ module phc_ref_driver
     solve_result, tracker_options, endgame_options, path_success,  &
     path_endgame_failed
   use polyhomocont__random, only : rng
+  use polyhomocont__system, only : poly_system
   use polyhomocont__linalg, only : lu_factor, lu_solve, cond1
   use polyhomocont__homotopy, only : total_degree_homotopy
   use polyhomocont__tracker, only : path_info, track_path, track_segment,  &
@@ -81,7 +82,37 @@ module phc_ref_driver
       singular_index, naccepted, nrejected
   end type c_path_results
 
+  !> A user evaluator over a sparse system: nvars, degrees and the affine
+  !> evaluate only.
+  type, extends(poly_system) :: affine_user_system
+    type(sparse_system) :: inner
+  contains
+    procedure :: nvars => au_nvars
+    procedure :: degrees => au_degrees
+    procedure :: evaluate => au_evaluate
+  end type affine_user_system
+
 contains
+
+  function au_nvars(self) result(n)
+    class(affine_user_system), intent(in) :: self
+    integer :: n
+    n = self%inner%nvars()
+  end function au_nvars
+
+  function au_degrees(self) result(d)
+    class(affine_user_system), intent(in) :: self
+    integer, allocatable :: d(:)
+    d = self%inner%degrees()
+  end function au_degrees
+
+  subroutine au_evaluate(self, x, f, jac)
+    class(affine_user_system), intent(inout) :: self
+    complex(wp), intent(in) :: x(:)
+    complex(wp), intent(out) :: f(:)
+    complex(wp), intent(out) :: jac(:, :)
+    call self%inner%evaluate(x, f, jac)
+  end subroutine au_evaluate
 
   subroutine build_system(csys, sys)
     !! add_equation for every equation of the flat system; statuses of
@@ -304,16 +335,45 @@ contains
     type(c_solve_summary), intent(inout) :: summ
     type(c_path_results), intent(in) :: paths
     integer(c_int) :: status
-
     type(sparse_system), target :: sys
     type(solve_options) :: opts
+
+    call build_system(csys, sys)
+    call to_options(copts, opts)
+    status = solve_export(sys, opts, summ, paths)
+  end function phc_ref_solve
+
+  function phc_ref_solve_user(csys, copts, summ, paths) result(status)  &
+      bind(C, name="phc_ref_solve_user")
+    !! The reference solve on the system wrapped as a user evaluator (only
+    !! the affine F and Jacobian; the default homogenization of
+    !! src/system.f90:73-117), the same wrapper as the library's
+    !! phc_f_solve_user.
+    type(c_user_system), intent(in) :: csys
+    type(c_solve_options), intent(in) :: copts
+    type(c_solve_summary), intent(inout) :: summ
+    type(c_path_results), intent(in) :: paths
+    integer(c_int) :: status
+    type(affine_user_system), target :: usr
+    type(solve_options) :: opts
+
+    call build_system(csys, usr%inner)
+    call to_options(copts, opts)
+    status = solve_export(usr, opts, summ, paths)
+  end function phc_ref_solve_user
+
+  function solve_export(sys, opts, summ, paths) result(status)
+    class(poly_system), intent(inout), target :: sys
+    type(solve_options), intent(in) :: opts
+    type(c_solve_summary), intent(inout) :: summ
+    type(c_path_results), intent(in) :: paths
+    integer(c_int) :: status
+
     type(solve_result) :: res
     integer(c_int32_t), pointer :: ip(:)
     real(c_double), pointer :: rp(:)
     integer :: st, k, np, n, j
 
-    call build_system(csys, sys)
-    call to_options(copts, opts)
     call solve(sys, res, st, opts)
     status = st
     if (st /= 0) return
@@ -404,7 +464,7 @@ contains
       a(k) = v
     end subroutine put_r
 
-  end function phc_ref_solve
+  end function solve_export
 
   function phc_ref_eval(csys, seed, z, t, h, hz, ht) result(status)  &
       bind(C, name="phc_ref_eval")

@@ -76,12 +76,12 @@ def build_fortran():
     objs = []
     for s in srcs:
         o = os.path.join(BUILD, os.path.basename(s)[:-4] + ".o")
-        _run(["flang", "-O2", "-fPIC", "-module-dir", mod, "-I", mod, "-c",
-              s, "-o", o])
+        _run(["flang", "-O2", "-fPIC", "-fopenmp", "-module-dir", mod, "-I",
+              mod, "-c", s, "-o", o])
         objs.append(o)
     out = os.path.join(LIB, "libpolyhomocont_amd.so")
     tmp = out + ".tmp"
-    _run(["flang", "-shared", "-fPIC", *objs, "-o", tmp,
+    _run(["flang", "-shared", "-fPIC", "-fopenmp", *objs, "-o", tmp,
           "-L" + LIB, "-lphc_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
     shutil.rmtree(BUILD, ignore_errors=True)

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstdarg>
 #include <chrono>
 #include <cmath>
@@ -50,13 +51,15 @@ struct phc_ctx {
 
 namespace {
 
+// Message of the last failed call made by this thread (any context).
+thread_local char t_err[512] = {0};
+
 int fail(phc_ctx* ctx, int code, const char* fmt, ...) {
-  if (ctx) {
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
-    va_end(ap);
-  }
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(t_err, sizeof(t_err), fmt, ap);
+  va_end(ap);
+  if (ctx) memcpy(ctx->err, t_err, sizeof(t_err));
   return code;
 }
 
@@ -100,9 +103,11 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
                       int64_t npaths, HostReq& h) {
   if (!sys || !hom || !o) return fail(ctx, PHC_ERR_INVALID_INPUT, "null input");
   const int n = sys->n, N = n + 1;
-  if (n < 1 || N > PHC_MAX_UNKNOWNS)
-    return fail(ctx, PHC_ERR_INVALID_INPUT, "n = %d outside 1..%d", n,
-                PHC_MAX_UNKNOWNS - 1);
+  if (n < 1) return fail(ctx, PHC_ERR_INVALID_INPUT, "n = %d < 1", n);
+  // a valid system the kernels cannot represent: not an input error
+  if (N > PHC_MAX_UNKNOWNS)
+    return fail(ctx, PHC_ERR_CAPACITY, "n = %d: the kernels hold at most %d "
+                "unknowns", n, PHC_MAX_UNKNOWNS - 1);
   if (sys->nterms < n || !sys->first || !sys->deg || !sys->hexps ||
       !sys->coefs)
     return fail(ctx, PHC_ERR_INVALID_INPUT, "incomplete system");
@@ -132,9 +137,13 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
     int tot = 0;
     for (int j = 0; j < N; ++j) {
       const int e = sys->hexps[(size_t)k * N + j];
-      if (e < 0 || e > 255)
+      if (e < 0)
         return fail(ctx, PHC_ERR_INVALID_INPUT,
-                    "exponent %d of term %d outside 0..255", e, k);
+                    "exponent %d of term %d is negative", e, k);
+      if (e > PHC_MAX_EXPONENT)
+        return fail(ctx, PHC_ERR_CAPACITY,
+                    "exponent %d of term %d: the kernels hold at most %d", e,
+                    k, PHC_MAX_EXPONENT);
       t.e[j >> 2] |= (uint32_t)e << (8 * (j & 3));
       maxexp = std::max(maxexp, e);
       tot += e;
@@ -181,8 +190,10 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
   r.closure = e.closure_factor;
   r.nsamples = e.nsamples;
   r.max_winding = e.max_winding;
-  if (r.use_endgame && (e.nsamples < 1 || e.nsamples > 4096))
-    return fail(ctx, PHC_ERR_INVALID_INPUT, "nsamples outside 1..4096");
+  if (r.use_endgame && e.nsamples < 1)
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "nsamples < 1");
+  if (r.use_endgame && e.nsamples > 4096)
+    return fail(ctx, PHC_ERR_CAPACITY, "nsamples %d above 4096", e.nsamples);
   // dtheta = 2 pi / nsamples and log(1/radius_factor) exactly as the
   // reference computes them (src/endgame.f90:350, :225)
   r.dtheta = 2.0 * 3.14159265358979323846 / (double)(e.nsamples > 0 ? e.nsamples : 1);
@@ -531,7 +542,8 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
   const int64_t row_waves = (std::max<int64_t>(cr.total, 1) * G + 63) / 64;
   const int64_t paths_needed = std::max<int64_t>(cr.total, 1);
   const size_t limit = std::min<size_t>(ctx->lds_limit, 160 * 1024);
-  for (int attempt = 0; attempt < 2; ++attempt) {
+  cr.blocks = 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
     if (grid)
       cr.mode = MODE_GRID;
     else if (can_split && (mapping == 2 ||
@@ -559,15 +571,22 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
              (paths_needed + threads / cr.lp - 1) / (threads / cr.lp) < 2 * cus)
         threads /= 2;
     }
-    // the grid unit keeps every path's term values in LDS: shrink the
-    // workgroup until it fits, and give up the grid mapping if one path
-    // does not
-    while (cr.mode == MODE_GRID && threads > cr.lp &&
-           (size_t)(threads / cr.lp) * per_path > limit)
+    // every path keeps its power table (and, on the grid, its term values)
+    // in LDS: shrink the workgroup until its paths fit; if one path does
+    // not, give up the grid or split mapping for the row mapping
+    while (threads > cr.lp && (size_t)(threads / cr.lp) * per_path > limit)
       threads /= 2;
-    if (cr.mode == MODE_GRID && (size_t)(threads / cr.lp) * per_path > limit) {
-      grid = false;
-      continue;
+    if ((size_t)(threads / cr.lp) * per_path > limit) {
+      if (cr.mode == MODE_GRID) {
+        grid = false;
+        continue;
+      }
+      if (cr.mode == MODE_SPLIT) {
+        can_split = false;
+        continue;
+      }
+      cr.blocks = 0;  // no mapping holds one path: the caller reports it
+      return;
     }
     const int per_wg = threads / cr.lp;
     int64_t blocks = (paths_needed + per_wg - 1) / per_wg;
@@ -601,6 +620,10 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
     cr.G = kv.first;
     cr.reqs = kv.second;
     plan_class(ctx, cr);
+    if (cr.blocks == 0)
+      return fail(ctx, PHC_ERR_CAPACITY,
+                  "one path's working set (power table of exponent %d, %d "
+                  "lanes) exceeds the LDS of a workgroup", cr.expmax, cr.G);
     for (HostReq* h : cr.reqs) {
       const int N = h->r.N;
       need += Arena::al(h->terms.size() * sizeof(TermRec));
@@ -853,6 +876,41 @@ int phc_ctx_lanes_per_path(const phc_ctx* ctx, int N) {
 const char* phc_ctx_last_error(const phc_ctx* ctx) {
   return ctx ? ctx->err : "no context";
 }
+
+const char* phc_thread_last_error(void) { return t_err; }
+
+int phc_check_system(const phc_system* sys, const phc_solve_options* opts) {
+  phc_homotopy hom;
+  static const double zero[2 * PHC_MAX_UNKNOWNS] = {0.0};
+  hom.gamma = zero;
+  hom.b = zero;
+  hom.patch = zero;
+  HostReq h;
+  return validate_and_pack(nullptr, sys, &hom, opts, 0, h);
+}
+
+int phc_ctx_shared(int device, phc_ctx** out) {
+  if (!out) return PHC_ERR_INVALID_INPUT;
+  *out = nullptr;
+  if (device < 0 || device >= 64)
+    return fail(nullptr, PHC_ERR_DEVICE, "device %d out of range", device);
+  static std::mutex mu;
+  static phc_ctx* shared[64] = {nullptr};
+  std::lock_guard<std::mutex> lk(mu);
+  if (!shared[device]) {
+    const int rc = phc_ctx_create(device, &shared[device]);
+    if (rc != PHC_OK) {
+      shared[device] = nullptr;
+      return fail(nullptr, rc, "no usable HIP device %d", device);
+    }
+  }
+  *out = shared[device];
+  return PHC_OK;
+}
+
+static std::atomic<int> g_default_device{0};
+void phc_set_default_device(int device) { g_default_device.store(device); }
+int phc_get_default_device(void) { return g_default_device.load(); }
 
 int phc_ctx_counters(const phc_ctx* ctx, uint64_t* out) {
   if (!ctx || !out) return PHC_ERR_INVALID_INPUT;

@@ -59,7 +59,15 @@ def lib():
         L.phc_server_progress.argtypes = [P]
         L.phc_server_poll.argtypes = [P, C.c_uint64]
         L.phc_server_wait.argtypes = [P, C.c_uint64]
+        L.phc_server_release.argtypes = [P, C.c_uint64]
+        L.phc_server_tickets.argtypes = [P, P]
         L.phc_server_destroy.argtypes = [P]
+        L.phc_thread_last_error.argtypes = []
+        L.phc_thread_last_error.restype = C.c_char_p
+        L.phc_ctx_shared.argtypes = [C.c_int, C.POINTER(P)]
+        L.phc_set_default_device.argtypes = [C.c_int]
+        L.phc_set_default_device.restype = None
+        L.phc_get_default_device.argtypes = []
         _lib = L
     return _lib
 
@@ -89,7 +97,8 @@ class Context:
 
     def _check(self, rc, what):
         if rc != 0:
-            msg = self.L.phc_ctx_last_error(self.h).decode(errors="replace")
+            # this thread's own message (another thread may share the context)
+            msg = self.L.phc_thread_last_error().decode(errors="replace")
             raise DeviceError(f"{what} failed with {rc}: {msg}", rc)
 
     def counters(self):
@@ -250,7 +259,19 @@ class Server:
         return self._keep[ticket][4]
 
     def release(self, ticket):
+        """Forgets a finished ticket (or cancels a queued one); the engine
+        keeps no record of it afterwards. Raises while its paths are on the
+        device, whose records are still to be copied into its arrays."""
+        self.ctx._check(self.ctx.L.phc_server_release(self.h, ticket),
+                        "phc_server_release")
         self._keep.pop(ticket, None)
+
+    def tickets(self):
+        """Tickets queued, admitted and finished-but-unreleased."""
+        c = np.zeros(3, np.int64)
+        self.ctx._check(self.ctx.L.phc_server_tickets(self.h, c.ctypes.data),
+                        "phc_server_tickets")
+        return dict(queued=int(c[0]), admitted=int(c[1]), finished=int(c[2]))
 
     def close(self):
         if self.h:

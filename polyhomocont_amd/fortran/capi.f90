@@ -4,8 +4,10 @@ module polyhomocont__capi
 !! benchmarks, the serving front end). Fortran callers use `polyhomocont`.
 
   use, intrinsic :: iso_c_binding
+  use omp_lib, only : omp_set_num_threads, omp_get_max_threads
   use polyhomocont__config, only : wp, status_ok
   use polyhomocont__random, only : rng
+  use polyhomocont__system, only : poly_system
   use polyhomocont__sparse, only : sparse_system
   use polyhomocont__homotopy, only : total_degree_homotopy
   use polyhomocont__solve, only : solve, solve_options, solve_result,  &
@@ -35,7 +37,42 @@ module polyhomocont__capi
       singular_index, naccepted, nrejected
   end type c_path_results
 
+  !> A system with a user-supplied evaluator: only the affine F(x) and its
+  !> Jacobian (here those of a sparse system), so solve homogenizes through
+  !> the default poly_system evaluate_homogeneous and tracks on the host,
+  !> as for the user types of the reference's tests (test/solve.f90:16-57).
+  !> `calls` makes it stateful, as user evaluators may be.
+  type, extends(poly_system) :: affine_user_system
+    type(sparse_system) :: inner
+    integer :: calls = 0
+  contains
+    procedure :: nvars => au_nvars
+    procedure :: degrees => au_degrees
+    procedure :: evaluate => au_evaluate
+  end type affine_user_system
+
 contains
+
+  function au_nvars(self) result(n)
+    class(affine_user_system), intent(in) :: self
+    integer :: n
+    n = self%inner%nvars()
+  end function au_nvars
+
+  function au_degrees(self) result(d)
+    class(affine_user_system), intent(in) :: self
+    integer, allocatable :: d(:)
+    d = self%inner%degrees()
+  end function au_degrees
+
+  subroutine au_evaluate(self, x, f, jac)
+    class(affine_user_system), intent(inout) :: self
+    complex(wp), intent(in) :: x(:)
+    complex(wp), intent(out) :: f(:)
+    complex(wp), intent(out) :: jac(:, :)
+    self%calls = self%calls + 1
+    call self%inner%evaluate(x, f, jac)
+  end subroutine au_evaluate
 
   subroutine build(csys, sys)
     !! add_equation for each equation of the flat description; rejected
@@ -217,6 +254,37 @@ contains
     if (st /= status_ok) return
     call export(res, summ, paths)
   end function phc_f_solve
+
+  function phc_f_solve_user(csys, copts, summ, paths) result(status)  &
+      bind(C, name="phc_f_solve_user")
+    !! solve on the system wrapped as a user evaluator (affine_user_system):
+    !! the host tracker over the OpenMP threads.
+    type(c_user_system), intent(in) :: csys
+    type(c_solve_options), intent(in) :: copts
+    type(c_solve_summary), intent(inout) :: summ
+    type(c_path_results), intent(in) :: paths
+    integer(c_int) :: status
+    type(affine_user_system), target :: usr
+    type(solve_result) :: res
+    integer :: st
+
+    call build(csys, usr%inner)
+    call solve(usr, res, st, from_c(copts))
+    status = st
+    if (st /= status_ok) return
+    call export(res, summ, paths)
+  end function phc_f_solve_user
+
+  function phc_f_set_threads(nthreads) result(previous)  &
+      bind(C, name="phc_f_set_threads")
+    !! Host threads of later solve calls made by the calling thread (the
+    !! OpenMP default when nthreads <= 0 is never set); returns the
+    !! previous count.
+    integer(c_int32_t), value :: nthreads
+    integer(c_int32_t) :: previous
+    previous = omp_get_max_threads()
+    if (nthreads > 0) call omp_set_num_threads(int(nthreads))
+  end function phc_f_set_threads
 
   function phc_f_homotopy(csys, seed, gamma, b, patch, nstart)  &
       result(status) bind(C, name="phc_f_homotopy")

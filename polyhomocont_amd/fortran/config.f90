@@ -18,5 +18,9 @@ module polyhomocont__config
   !> The GPU path could not run (no device, or a HIP error). Sparse systems
   !> are never tracked on the CPU instead.
   integer, parameter, public :: err_device = -1
+  !> Internal: a valid sparse system the GPU kernels cannot hold (more than
+  !> 15 variables or an exponent above 255); solve tracks it on the host.
+  !> Never returned by solve.
+  integer, parameter, public :: err_capacity = -2
 
 end module polyhomocont__config

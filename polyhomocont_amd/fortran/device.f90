@@ -3,13 +3,15 @@ module polyhomocont__device
 !! mirrors of the C structs and the calls `solve` makes to track the paths
 !! of a sparse_system on the GPU.
 !!
-!! One device context per process is created on first use (device index
-!! from set_device, default 0) and reused by every later solve; the C
-!! library serializes concurrent calls on it.
+!! The device context is the C library's process-wide context of the
+!! device (phc_ctx_shared: created once per device under the library's
+!! lock, kept until the process ends), so this module holds no mutable
+!! state and `solve` stays reentrant. The device index is the library's
+!! atomic default (set_device; -1 asks solve to track on the host).
 
   use, intrinsic :: iso_c_binding
   use polyhomocont__config, only : wp, status_ok, err_invalid_input,  &
-    err_device
+    err_device, err_capacity
   use polyhomocont__sparse, only : sparse_system
   use polyhomocont__homotopy, only : total_degree_homotopy
   use polyhomocont__tracker, only : tracker_options
@@ -21,7 +23,7 @@ module polyhomocont__device
 
   public :: c_tracker_options, c_endgame_options, c_solve_options
   public :: c_endpoints, c_system, c_homotopy
-  public :: device_track, set_device, device_error_message
+  public :: device_track, set_device, device_error_message, host_requested
   public :: endpoint_records
 
   type, bind(C) :: c_tracker_options
@@ -67,13 +69,23 @@ module polyhomocont__device
   end type endpoint_records
 
   interface
-    function phc_ctx_create(device, ctx) result(rc)  &
-        bind(C, name="phc_ctx_create")
+    function phc_ctx_shared(device, ctx) result(rc)  &
+        bind(C, name="phc_ctx_shared")
       import :: c_int, c_ptr
       integer(c_int), value :: device
       type(c_ptr), intent(out) :: ctx
       integer(c_int) :: rc
-    end function phc_ctx_create
+    end function phc_ctx_shared
+    subroutine phc_set_default_device(device)  &
+        bind(C, name="phc_set_default_device")
+      import :: c_int
+      integer(c_int), value :: device
+    end subroutine phc_set_default_device
+    function phc_get_default_device() result(device)  &
+        bind(C, name="phc_get_default_device")
+      import :: c_int
+      integer(c_int) :: device
+    end function phc_get_default_device
     function phc_track_paths(ctx, sys, hom, opts, npaths, start, out)  &
         result(rc) bind(C, name="phc_track_paths")
       import :: c_ptr, c_int, c_int64_t, c_system, c_homotopy,  &
@@ -87,33 +99,44 @@ module polyhomocont__device
       type(c_endpoints), intent(in) :: out
       integer(c_int) :: rc
     end function phc_track_paths
-    function phc_ctx_last_error(ctx) result(msg)  &
-        bind(C, name="phc_ctx_last_error")
+    function phc_check_system(sys, opts) result(rc)  &
+        bind(C, name="phc_check_system")
+      import :: c_int, c_system, c_solve_options
+      type(c_system), intent(in) :: sys
+      type(c_solve_options), intent(in) :: opts
+      integer(c_int) :: rc
+    end function phc_check_system
+    function phc_thread_last_error() result(msg)  &
+        bind(C, name="phc_thread_last_error")
       import :: c_ptr
-      type(c_ptr), value :: ctx
       type(c_ptr) :: msg
-    end function phc_ctx_last_error
+    end function phc_thread_last_error
   end interface
-
-  type(c_ptr), save :: ctx = c_null_ptr
-  integer, save :: ctx_device = -1
-  integer, save :: wanted_device = 0
 
 contains
 
   subroutine set_device(device)
+    !! HIP device of later solve calls (default 0). -1 tracks sparse
+    !! systems on the host CPU instead, by explicit request only: solve
+    !! never falls back to the host when the device it was asked for fails.
     integer, intent(in) :: device
-    wanted_device = device
+    call phc_set_default_device(int(device, c_int))
   end subroutine set_device
 
+  function host_requested() result(yes)
+    logical :: yes
+    yes = phc_get_default_device() < 0
+  end function host_requested
+
   function device_error_message() result(msg)
+    !! Message of the calling thread's last failed library call.
     character(len=:), allocatable :: msg
     character(kind=c_char), pointer :: p(:)
     type(c_ptr) :: cp
     integer :: k
     msg = ""
-    if (.not. c_associated(ctx)) return
-    cp = phc_ctx_last_error(ctx)
+    cp = phc_thread_last_error()
+    if (.not. c_associated(cp)) return
     call c_f_pointer(cp, p, [512])
     do k = 1, 512
       if (p(k) == c_null_char) exit
@@ -121,17 +144,14 @@ contains
     end do
   end function device_error_message
 
-  function get_context() result(rc)
+  function get_context(ctx) result(rc)
+    type(c_ptr), intent(out) :: ctx
     integer :: rc
-    rc = status_ok
-    if (c_associated(ctx) .and. ctx_device == wanted_device) return
-    rc = phc_ctx_create(int(wanted_device, c_int), ctx)
+    rc = phc_ctx_shared(phc_get_default_device(), ctx)
     if (rc /= 0) then
       ctx = c_null_ptr
       rc = err_device
-      return
     end if
-    ctx_device = wanted_device
   end function get_context
 
   function to_c_options(topts, eopts, use_endgame) result(o)
@@ -161,8 +181,9 @@ contains
       status)
     !! Tracks every path whose start point is a column of `start` (N x
     !! npaths) on the GPU; rec receives one endpoint record per path.
-    !! status: status_ok, err_invalid_input (rejected by the library) or
-    !! err_device.
+    !! status: status_ok, err_invalid_input (rejected by the library),
+    !! err_capacity (a valid system the kernels cannot hold: more than 15
+    !! variables or an exponent above 255) or err_device.
     type(sparse_system), intent(in) :: sys
     type(total_degree_homotopy), intent(in) :: hom
     type(tracker_options), intent(in) :: topts
@@ -179,9 +200,8 @@ contains
     type(c_solve_options) :: co
     type(c_endpoints) :: ce
     integer :: n, nt, np, rc
+    type(c_ptr) :: ctx
 
-    status = get_context()
-    if (status /= status_ok) return
     n = sys%nvars()
     nt = sys%total_terms()
     np = size(start, 2)
@@ -194,6 +214,14 @@ contains
     patch = hom%patch
     ch = c_homotopy(c_loc(gam), c_loc(b), c_loc(patch))
     co = to_c_options(topts, eopts, use_endgame)
+    ! systems the kernels cannot hold never need the device
+    rc = phc_check_system(cs, co)
+    if (rc == -2) then  ! PHC_ERR_CAPACITY
+      status = err_capacity
+      return
+    end if
+    status = get_context(ctx)
+    if (status /= status_ok) return
 
     allocate(rec%status(np), rec%winding(np), rec%eg_status(np),  &
       rec%naccepted(np), rec%nrejected(np), rec%t(np), rec%eg_error(np),  &
@@ -219,6 +247,8 @@ contains
       status = status_ok
     else if (rc > 0) then
       status = err_invalid_input
+    else if (rc == -2) then  ! PHC_ERR_CAPACITY
+      status = err_capacity
     else
       status = err_device
     end if

@@ -28,6 +28,9 @@ class HostLibrary:
         L.phc_f_classify.argtypes = [P, P, i64, P, P, P]
         L.phc_f_set_device.argtypes = [i32]
         L.phc_f_set_device.restype = None
+        L.phc_f_solve_user.argtypes = [P, P, P, P]
+        L.phc_f_set_threads.argtypes = [i32]
+        L.phc_f_set_threads.restype = i32
 
     def solve(self, usys, opts):
         """(status, SolveArrays) of `solve` on this system."""
@@ -35,6 +38,19 @@ class HostLibrary:
         st = self.lib.phc_f_solve(C.byref(usys.c), C.byref(opts),
                                   C.byref(res.summary), C.byref(res.paths))
         return st, res
+
+    def solve_user(self, usys, opts):
+        """(status, SolveArrays) of `solve` on the system wrapped as a user
+        evaluator (host tracker, OpenMP threads)."""
+        res = abi.SolveArrays(max(usys.bezout, 1), usys.n)
+        st = self.lib.phc_f_solve_user(C.byref(usys.c), C.byref(opts),
+                                       C.byref(res.summary), C.byref(res.paths))
+        return st, res
+
+    def set_threads(self, nthreads):
+        """Host threads of this thread's later solves; returns the previous
+        count."""
+        return int(self.lib.phc_f_set_threads(nthreads))
 
     def homotopy(self, usys, seed):
         n = usys.n
