@@ -321,6 +321,8 @@ int phc_server_wait(phc_server *srv, uint64_t ticket);
  * are on the device; PHC_ERR_INVALID_INPUT for an unknown ticket. A
  * long-running server calls it once per request, after poll/wait. */
 int phc_server_release(phc_server *srv, uint64_t ticket);
+/* Most paths one request may have (the smallest class's ring capacity). */
+int64_t phc_server_max_request_paths(const phc_server *srv);
 /* counts[0] queued, [1] admitted, [2] finished and not yet released. */
 int phc_server_tickets(phc_server *srv, int64_t *counts);
 int phc_server_destroy(phc_server *srv);

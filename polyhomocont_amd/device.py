@@ -61,6 +61,8 @@ def lib():
         L.phc_server_wait.argtypes = [P, C.c_uint64]
         L.phc_server_release.argtypes = [P, C.c_uint64]
         L.phc_server_tickets.argtypes = [P, P]
+        L.phc_server_max_request_paths.argtypes = [P]
+        L.phc_server_max_request_paths.restype = i64
         L.phc_server_destroy.argtypes = [P]
         L.phc_thread_last_error.argtypes = []
         L.phc_thread_last_error.restype = C.c_char_p
@@ -265,6 +267,10 @@ class Server:
         self.ctx._check(self.ctx.L.phc_server_release(self.h, ticket),
                         "phc_server_release")
         self._keep.pop(ticket, None)
+
+    def max_request_paths(self):
+        """Most paths one request may have."""
+        return int(self.ctx.L.phc_server_max_request_paths(self.h))
 
     def tickets(self):
         """Tickets queued, admitted and finished-but-unreleased."""

@@ -15,7 +15,15 @@ One thread runs the whole loop: it accepts connections, reads frames,
 submits requests as they arrive and advances the engine while any are in
 flight. Replies go out in completion order, tagged with the client's
 request id.
+
+Limits keep one client from stalling the others: a request whose path
+count (the product of its equations' degrees, known before anything is
+built) exceeds `max_paths` -- by default the engine's ring capacity -- is
+refused with a capacity error before its start points are drawn; a frame
+header announcing more than `max_frame_bytes` ends that connection; any
+failure while handling one request becomes that request's error reply.
 """
+import math
 import selectors
 import socket
 
@@ -23,9 +31,9 @@ from . import abi, device, wire
 
 
 class _Conn:
-    def __init__(self, sock):
+    def __init__(self, sock, max_frame_bytes):
         self.sock = sock
-        self.reader = wire.FrameReader()
+        self.reader = wire.FrameReader(max_frame_bytes)
         self.out = bytearray()
         self.closed = False
 
@@ -36,9 +44,14 @@ class FrontEnd:
     object with its submit/progress/done/release methods); `host_lib` the
     Fortran host library that builds and checks systems."""
 
-    def __init__(self, server, host_lib, address=("127.0.0.1", 0)):
+    def __init__(self, server, host_lib, address=("127.0.0.1", 0),
+                 max_paths=None, max_frame_bytes=64 << 20):
         self.srv = server
         self.H = host_lib
+        if max_paths is None:
+            max_paths = getattr(server, "max_request_paths", lambda: 1 << 20)()
+        self.max_paths = int(max_paths)
+        self.max_frame_bytes = int(max_frame_bytes)
         self.listener = socket.create_server(address)
         self.listener.setblocking(False)
         self.address = self.listener.getsockname()
@@ -70,6 +83,14 @@ class FrontEnd:
                                                 str(e)))
             return
         try:
+            degrees = usys.degrees
+            paths = math.prod(degrees) if degrees and min(degrees) > 0 else 0
+            if paths > self.max_paths:
+                self.rejected += 1
+                self._reply(conn, wire.encode_error(
+                    rid, wire.PHC_ERR_CAPACITY,
+                    f"{paths} paths exceed the server's {self.max_paths}"))
+                return
             req = self._prepare(usys, opts)
             ticket, out = self.srv.submit(*req)
         except ValueError as e:
@@ -80,6 +101,11 @@ class FrontEnd:
         except device.DeviceError as e:
             self.rejected += 1
             self._reply(conn, wire.encode_error(rid, e.code or -1, str(e)))
+            return
+        except Exception as e:  # noqa: BLE001 -- one request, not the loop
+            self.rejected += 1
+            self._reply(conn, wire.encode_error(
+                rid, -1, f"request failed: {type(e).__name__}: {e}"))
             return
         self.pending[ticket] = (conn, rid, len(req[3]), out)
 
@@ -149,7 +175,7 @@ class FrontEnd:
                     continue
                 s.setblocking(False)
                 s.setsockopt(socket.IPPROTO_TCP, socket.TCP_NODELAY, 1)
-                c = _Conn(s)
+                c = _Conn(s, self.max_frame_bytes)
                 self.sel.register(s, selectors.EVENT_READ, c)
                 continue
             conn = key.data

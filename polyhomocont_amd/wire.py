@@ -24,6 +24,7 @@ HEADER_BYTES = 32
 PEEK_BYTES = 16
 MAX_FRAME = 1 << 30
 PHC_ERR_INVALID_INPUT = 1
+PHC_ERR_CAPACITY = -2
 
 _lib = None
 
@@ -202,18 +203,25 @@ def peek(head):
 
 
 class FrameReader:
-    """Splits a byte stream into whole frames."""
+    """Splits a byte stream into whole frames, each at most `max_bytes`
+    long (a longer frame is refused from its header, before its body is
+    buffered)."""
 
-    def __init__(self):
+    def __init__(self, max_bytes=1 << 30):
         self.buf = bytearray()
+        self.max_bytes = int(max_bytes)
 
     def feed(self, data):
         """Appends bytes; returns the frames completed by them. Raises
-        WireError on a stream that is not a frame sequence."""
+        WireError on a stream that is not a frame sequence or announces a
+        frame above the limit."""
         self.buf += data
         frames = []
         while len(self.buf) >= PEEK_BYTES:
             _, n = peek(self.buf)
+            if n > self.max_bytes:
+                raise WireError(f"frame of {n} bytes above the limit of "
+                                f"{self.max_bytes}")
             if len(self.buf) < n:
                 break
             frames.append(bytes(self.buf[:n]))

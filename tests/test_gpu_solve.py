@@ -85,3 +85,45 @@ def test_c1_full_and_c2_counts(host):
         assert r["nsolutions"] == npaths and r["nduplicates"] == 0
         assert r["nfailed"] == r["ninfinite"] == r["nsingular"] == 0
         assert res.p["residual"][:npaths].max() < 1e-12
+
+
+def test_concurrent_solves_share_the_device(host):
+    """solve is reentrant on the GPU path: two threads solving at once on
+    the library's shared device context get the results of solving one
+    after the other (src/solve.f90 keeps no module state; CLAUDE.md of the
+    reference asks the same of a parallel caller)."""
+    import threading
+    systems = [W.ref_test_systems()["katsura4"], W.config_system("c1"),
+               W.ref_test_systems()["cyclic5"]]
+    opts = abi.default_options()
+    alone = []
+    for u in systems:
+        st, r = host.solve(u, opts)
+        assert st == 0
+        alone.append(r)
+    got = [[] for _ in systems]
+    errors = []
+
+    def work(i):
+        try:
+            for _ in range(3):
+                st, r = host.solve(systems[i], opts)
+                assert st == 0
+                got[i].append(r)
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    print("[gpu] concurrent solves from 3 threads", flush=True)
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(systems))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    for rs, a in zip(got, alone):
+        for r in rs:
+            assert r.counts() == a.counts()
+            n = a.summary.npaths
+            for f in ("status", "winding", "solution_index", "singular_index"):
+                assert np.array_equal(r.p[f][:n], a.p[f][:n]), f
+            assert np.array_equal(r.p["z"][:n], a.p["z"][:n])

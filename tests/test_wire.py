@@ -290,3 +290,56 @@ def test_front_end_protocol(host):
         stop.set()
         th.join(timeout=30)
         fe.close()
+
+
+class FailingEngine(StandInEngine):
+    """Raises an unexpected error for requests with seed 13."""
+
+    def submit(self, packed, hom, opts, start):
+        if opts.seed == 13:
+            raise RuntimeError("engine fault")
+        return super().submit(packed, hom, opts, start)
+
+
+def test_front_end_limits_keep_serving(host):
+    """A request with more paths than the engine takes is refused from its
+    degrees (no start points are built), an unexpected failure of one
+    request becomes its error reply, a frame above the size limit ends only
+    its own connection -- and the loop keeps serving other requests."""
+    from polyhomocont_amd import frontend
+    fe = frontend.FrontEnd(FailingEngine(), host, max_paths=100,
+                           max_frame_bytes=16384)
+    stop = threading.Event()
+    th = threading.Thread(target=fe.serve, args=(stop,), daemon=True)
+    th.start()
+    try:
+        s = _systems()
+        big = W.dense_system(5, 3, 9)                 # 243 paths > 100
+        cl = frontend.Client(fe.address, timeout=60)
+        cl.send(big, abi.default_options(), 1)
+        cl.send(s["c1"], abi.default_options(seed=13), 2)
+        cl.send(s["c1"], abi.default_options(seed=4), 3)
+        replies = {}
+        for _ in range(3):
+            rid, rec, err, msg = cl.receive()
+            replies[rid] = (rec, err, msg)
+        assert replies[1][1] == wire.PHC_ERR_CAPACITY and "243" in replies[1][2]
+        assert replies[2][1] == -1 and "engine fault" in replies[2][2]
+        assert replies[3][1] == 0 and replies[3][0].npaths == 16
+        # an oversized frame: one error reply, then that connection closes
+        raw = socket.create_connection(fe.address, timeout=60)
+        raw.sendall(wire.encode_request(W.dense_system(7, 3, 5),
+                                        abi.default_options(), 9))
+        f = wire.recv_frame(raw)
+        assert wire.decode_endpoints(f)[2] == 1
+        assert wire.recv_frame(raw) is None
+        raw.close()
+        # the first client is still served
+        cl.send(s["c1"], abi.default_options(seed=2), 4)
+        rid, rec, err, _ = cl.receive()
+        assert rid == 4 and err == 0
+        cl.close()
+    finally:
+        stop.set()
+        th.join(timeout=30)
+        fe.close()
