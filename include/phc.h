@@ -240,6 +240,28 @@ int phc_track_paths(phc_ctx *ctx, const phc_system *sys,
                     const phc_homotopy *hom, const phc_solve_options *opts,
                     int64_t npaths, const double *start, phc_endpoints *out);
 
+/* Allocation-free variant of phc_track_paths, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * `d_start` ([npaths][N][2] doubles) and every array of `d_out` (npaths
+ * entries; z [npaths][N][2]; eg_residual, nlin, nbe may be NULL) are
+ * DEVICE memory the caller owns on the context's device, ready before the
+ * call (synchronize the stream that wrote them). The library calls no
+ * hipMalloc/hipFree: the request record, the term and grid tables and the
+ * counters of the launch live in a static device array of
+ * phc_noalloc_scratch_bytes() bytes (1 MiB) that each GPU holds until the
+ * process ends; a system whose tables do not fit returns PHC_ERR_CAPACITY
+ * (C2's 840 terms need about 100 KB). Calls on one GPU share that array:
+ * the library serializes them per device within the process (the array
+ * keeps the last call's contents), and a caller that ran two of them at
+ * once from outside that serialization would race. Synchronous; results
+ * are those of phc_track_paths, bit for bit. */
+int phc_track_paths_noalloc(phc_ctx *ctx, const phc_system *sys,
+                            const phc_homotopy *hom,
+                            const phc_solve_options *opts, int64_t npaths,
+                            const double *d_start, phc_endpoints *d_out);
+int64_t phc_noalloc_scratch_bytes(void);
+
 /* Same, with the system, constants, start points and outputs already in
  * device memory prepared by phc_stage_* below (used for timing with inputs
  * resident in HBM). Asynchronous on the context stream; phc_ctx_sync waits. */

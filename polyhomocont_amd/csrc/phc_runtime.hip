@@ -802,6 +802,50 @@ int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
   return PHC_OK;
 }
 
+// ------------------------------------------------- allocation-free scratch
+// Device memory of phc_track_paths_noalloc beyond its caller's inputs and
+// outputs: the request record, the term and grid tables, the work-queue
+// head, the path-start offsets and the unit counters of one launch. Every
+// GPU the library runs on holds one copy of this array until the process
+// ends. Host code never passes `scratch` itself to a kernel or a copy (in
+// host code the name is a host-side stand-in, not a device address): the
+// device address comes from hipGetSymbolAddress, once per device, at the
+// first context created on it.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)1 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;                          // guards g_noalloc_ptr
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {nullptr};
+std::mutex g_noalloc_run[NOALLOC_MAX_DEVICES];    // one call per device
+
+// Looks up this device's copy of `scratch` (the device must be current).
+int noalloc_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return PHC_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[device]) return PHC_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(nullptr, PHC_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s",
+                hipGetErrorString(e));
+  g_noalloc_ptr[device] = ptr;
+  return PHC_OK;
+}
+
+void* noalloc_ptr(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  return g_noalloc_ptr[device];
+}
+
+// Clears `words` 32-bit words (a kernel, not hipMemsetAsync, so that the
+// clearing is an ordinary kernel node if a caller captures the stream).
+__global__ void zero_words(uint32_t* p, int words) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) p[i] = 0u;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -856,6 +900,9 @@ int phc_ctx_create(int device, phc_ctx** out) {
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
       hipSuccess)
     return PHC_ERR_DEVICE;
+  // this device's scratch of phc_track_paths_noalloc; a failure here only
+  // disables that entry point (it reports the error when called)
+  (void)noalloc_setup(device);
   *out = ctx.release();
   return PHC_OK;
 }
@@ -1129,6 +1176,121 @@ int phc_track_batch(phc_ctx* ctx, int32_t nsys, const phc_system* sys,
   rc = run_classes(ctx, st.get());
   if (rc) return rc;
   return fetch_outputs(ctx, st.get());
+}
+
+int phc_track_paths_noalloc(phc_ctx* ctx, const phc_system* sys,
+                            const phc_homotopy* hom,
+                            const phc_solve_options* opts, int64_t npaths,
+                            const double* d_start, phc_endpoints* d_out) {
+  if (!ctx) return PHC_ERR_DEVICE;
+  if (npaths == 0) return PHC_OK;
+  if (!d_start || !d_out || !d_out->status || !d_out->t || !d_out->z ||
+      !d_out->winding || !d_out->eg_status || !d_out->eg_error ||
+      !d_out->naccepted || !d_out->nrejected)
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "null device input or output");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  char* const base = (char*)noalloc_ptr(ctx->device);
+  if (!base)
+    return fail(ctx, PHC_ERR_DEVICE, "no scratch address for device %d",
+                ctx->device);
+  std::lock_guard<std::mutex> run_lk(g_noalloc_run[ctx->device]);
+  HostReq h;
+  int rc = validate_and_pack(ctx, sys, hom, opts, npaths, h);
+  if (rc) return rc;
+  ClassRun cr;
+  cr.G = h.G;
+  cr.reqs.push_back(&h);
+  plan_class(ctx, cr);
+  if (cr.blocks == 0)
+    return fail(ctx, PHC_ERR_CAPACITY,
+                "one path's working set exceeds the LDS of a workgroup");
+  // lay out the scratch: counters (cleared), queue head, offsets, request,
+  // then the tables
+  size_t used = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base + used;
+    used += Arena::al(bytes);
+    return p;
+  };
+  unsigned long long* d_next = (unsigned long long*)take(8);
+  unsigned long long* d_counters = (unsigned long long*)take(32);
+  const size_t clear_bytes = used;
+  int64_t* d_base = (int64_t*)take(2 * sizeof(int64_t));
+  Req* d_req = (Req*)take(sizeof(Req));
+  TermRec* d_terms = (TermRec*)take(h.terms.size() * sizeof(TermRec));
+  WTerm* d_wterms = h.wterms.empty()
+                        ? nullptr
+                        : (WTerm*)take(h.wterms.size() * sizeof(WTerm));
+  uint4* d_jobs =
+      h.jobs.empty() ? nullptr : (uint4*)take(h.jobs.size() * sizeof(uint4));
+  if (used > NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, PHC_ERR_CAPACITY,
+                "the system's tables need %zu bytes of scratch; the "
+                "allocation-free entry point holds %zu",
+                used, NOALLOC_SCRATCH_BYTES);
+  Req& r = h.r;
+  r.terms = d_terms;
+  r.wterms = cr.mode == MODE_GRID ? d_wterms : nullptr;
+  r.jobs = cr.mode == MODE_GRID ? d_jobs : nullptr;
+  r.start = (const cx*)d_start;
+  r.o_status = d_out->status;
+  r.o_t = d_out->t;
+  r.o_z = (cx*)d_out->z;
+  r.o_wind = d_out->winding;
+  r.o_egs = d_out->eg_status;
+  r.o_egerr = d_out->eg_error;
+  r.o_egres = d_out->eg_residual;
+  r.o_acc = d_out->naccepted;
+  r.o_rej = d_out->nrejected;
+  r.o_nlin = d_out->nlin;
+  r.o_nbe = d_out->nbe;
+  r.o_done = nullptr;
+  const int64_t offs[2] = {0, npaths};
+  hipStream_t s = ctx->stream;
+  const int words = (int)(clear_bytes / 4);
+  hipLaunchKernelGGL(zero_words, dim3((words + 255) / 256), dim3(256), 0, s,
+                     (uint32_t*)base, words);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(d_base, offs, sizeof(offs),
+                             hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(d_req, &r, sizeof(Req), hipMemcpyHostToDevice, s));
+  HIPCHK(ctx, hipMemcpyAsync(d_terms, h.terms.data(),
+                             h.terms.size() * sizeof(TermRec),
+                             hipMemcpyHostToDevice, s));
+  if (r.wterms)
+    HIPCHK(ctx, hipMemcpyAsync(d_wterms, h.wterms.data(),
+                               h.wterms.size() * sizeof(WTerm),
+                               hipMemcpyHostToDevice, s));
+  if (r.jobs)
+    HIPCHK(ctx, hipMemcpyAsync(d_jobs, h.jobs.data(),
+                               h.jobs.size() * sizeof(uint4),
+                               hipMemcpyHostToDevice, s));
+  Launch L;
+  memset(&L, 0, sizeof(L));
+  L.reqs = d_req;
+  L.req_base = d_base;
+  L.nreq = 1;
+  L.expmax = cr.expmax;
+  L.vmax = cr.vmax;
+  L.next = d_next;
+  L.total = npaths;
+  L.counters = d_counters;
+  HIPCHK(ctx, launch_g(cr.G, L, cr.staged, cr.blocks, cr.threads, cr.lds, s,
+                       cr.mode));
+  unsigned long long c[4] = {0, 0, 0, 0};
+  HIPCHK(ctx, hipMemcpyAsync(c, d_counters, sizeof(c), hipMemcpyDeviceToHost,
+                             s));
+  // the host tables above must outlive the copies: the call is synchronous
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  ctx->counters[0] = ctx->counters[3] = 0;
+  ctx->counters[1] = c[1];
+  ctx->counters[2] = c[2];
+  return PHC_OK;
+}
+
+int64_t phc_noalloc_scratch_bytes(void) {
+  return (int64_t)NOALLOC_SCRATCH_BYTES;
 }
 
 }  // extern "C"
