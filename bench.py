@@ -1,23 +1,40 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X path tracker.
 
-Workload (one step): one solve of C2 -- a single random dense system,
-n = 7, every equation of total degree 3 with all 120 monomials of degree
-<= 3, complex-Gaussian coefficients (seed 102), complex128 -- i.e. all
-2187 total-degree paths tracked from t = 1 through the Cauchy end game by
-the plain per-solve GPU tracker. Inputs (packed system, constants, start
-points) are resident in HBM before the timed region; the timed region is K
-back-to-back tracking runs.
+Headline workload (one step): the serving stream -- 512 solve requests,
+each drawn at random with equal odds from the random sparse generator (n
+3..8, degree 2..4, 5..40 terms per equation) or as a dense system (n 2..7,
+degree 2..3, all monomials), kept when they have 8..4096 total-degree paths
+(239,777 paths in all; seed 105), complex128 -- served once by the
+continuous-batching engine with every request queued at the start. A
+request is done when its classified result is on the host: homotopy
+constants and start points are drawn, the paths tracked on the GPU, the
+endpoints classified, refined and deduplicated on the host (host work runs
+on a pool of host threads beside the GPU). `value` is paths/s over the K
+timed passes.
 
-Reported beside the throughput:
-  roofline      FP64 work of the tracking kernel: canonical flops per
-                launch (polyhomocont_amd/flops.py, per-path linearization
-                counts from the device) over the kernel's average duration
-                measured with HIP events on the library's stream;
-  cpu_baseline  the compiled reference (oracle/_ref, flang -O2) on one host
-                core over a bounded sample of the same paths;
-  parity        endpoints of that sample vs the reference (max relative
-                difference; classification fields compared exactly).
+Reported beside it:
+  plain         the same stream served by the plain per-solve path: `solve`
+                (Fortran API: constants, start points, one GPU tracking
+                launch, classification) called once per request, in order;
+  latency       p50/p99 per-request latency (arrival -> classified result)
+                under Poisson arrivals at 70% of the plain path's saturation
+                rate, for the engine and for the plain path, and a sweep of
+                arrival rates;
+  roofline      FP64 work of the engine's dominant kernel (the lanes class
+                with the most kernel time): canonical flops (flops.py, per
+                path linearization counts from the device) over its kernel
+                time, measured with HIP events around each round on the
+                class's stream;
+  cpu_baseline  the compiled reference `solve` (oracle/_ref, flang -O2) on
+                one host core over the first requests of the same stream;
+  parity        engine results vs the plain path's (every request) and vs
+                the reference's (the cpu_baseline sample);
+  c2            the plain tracker on one dense n = 7, degree 3 system
+                (2187 paths) with inputs resident in HBM, its roofline and
+                CPU baseline;
+  batch         the 512 seeded random sparse systems in one launch per lanes
+                class.
 
 --gpus N > 1 is not supported: the tracker runs on one GPU.
 """
@@ -25,13 +42,15 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix), dense
-PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector, dense
+UNITS_PER_LAUNCH = 64
 
 
 def log(msg):
@@ -51,256 +70,429 @@ def load_traffic(name):
         return None
 
 
-def cpu_baseline(usys, opts, npaths, budget_s):
+def percentiles(xs):
+    import numpy as np
+    xs = np.asarray(xs) * 1e3
+    return {"p50": round(float(np.percentile(xs, 50)), 3),
+            "p90": round(float(np.percentile(xs, 90)), 3),
+            "p99": round(float(np.percentile(xs, 99)), 3),
+            "max": round(float(xs.max()), 3),
+            "mean": round(float(xs.mean()), 3)}
+
+
+# ------------------------------------------------------------------ serving
+def prepare(H, usys, opts):
+    """What `solve` does before tracking (src/solve.f90:166-190)."""
+    from polyhomocont_amd import abi
+    g, b, a, _ = H.homotopy(usys, opts.seed)
+    return (H.pack(usys), abi.HomotopyData(g, b, a), opts,
+            H.start_points(usys, opts.seed))
+
+
+class Engine:
+    """The continuous-batching engine with a host pool for the host halves
+    of `solve` (constants and start points before, classification after).
+    serve() runs a stream and returns per-request results and latencies."""
+
+    def __init__(self, ctx, H, workers=4, host_threads=4, resident=16384,
+                 budget=2 << 30, units=UNITS_PER_LAUNCH):
+        from polyhomocont_amd import device
+        self.H = H
+        self.srv = device.Server(ctx, max_resident_paths=resident,
+                                 hbm_budget_bytes=budget,
+                                 units_per_launch=units)
+        self.pool = ThreadPoolExecutor(
+            workers, initializer=lambda: H.set_threads(host_threads))
+
+    def close(self):
+        self.pool.shutdown()
+        self.srv.close()
+
+    def serve(self, stream, opts, arrivals=None, timeout=600.0):
+        R = len(stream)
+        prep, inflight, classify = {}, {}, {}
+        results, lat, cls, recs = [None] * R, [None] * R, [None] * R, [None] * R
+        arr = arrivals if arrivals is not None else [0.0] * R
+        nxt, done = 0, 0
+        t0 = time.perf_counter()
+        while done < R:
+            now = time.perf_counter() - t0
+            if now > timeout:
+                raise RuntimeError("serving run did not finish")
+            while nxt < R and arr[nxt] <= now:
+                prep[nxt] = self.pool.submit(prepare, self.H, stream[nxt], opts)
+                nxt += 1
+            for i in [i for i, f in prep.items() if f.done()]:
+                req = prep.pop(i).result()
+                t, out = self.srv.submit(*req)
+                cls[i] = self.srv.ticket_class(t)
+                inflight[t] = (i, out, req[0])
+            if inflight:
+                self.srv.progress()
+                for t in [t for t in inflight if self.srv.done(t)]:
+                    i, out, packed = inflight.pop(t)
+                    self.srv.release(t)
+                    recs[i] = (packed, out)
+                    classify[self.pool.submit(self.H.classify, stream[i], opts,
+                                              out)] = i
+            for f in [f for f in classify if f.done()]:
+                i = classify.pop(f)
+                st, res = f.result()
+                if st != 0:
+                    raise RuntimeError(f"classification of request {i}: {st}")
+                results[i] = res
+                lat[i] = time.perf_counter() - t0 - arr[i]
+                done += 1
+            if not inflight:
+                # host work only (or nothing yet): wait a little, at most
+                # until the next arrival
+                wait = (arr[nxt] - (time.perf_counter() - t0)) if nxt < R else 0
+                cap = 0.0005 if (prep or classify) else 0.05
+                time.sleep(min(max(wait, 0.0), cap) if wait > 0 or not
+                           (prep or classify) else cap)
+        return results, lat, cls, recs, time.perf_counter() - t0
+
+
+def serve_plain(H, stream, opts, arrivals=None):
+    """The plain per-solve path: `solve` per request, in arrival order."""
+    R = len(stream)
+    results, lat = [None] * R, [None] * R
+    arr = arrivals if arrivals is not None else [0.0] * R
+    t0 = time.perf_counter()
+    for i, usys in enumerate(stream):
+        now = time.perf_counter() - t0
+        if now < arr[i]:
+            time.sleep(arr[i] - now)
+        st, res = H.solve(usys, opts)
+        if st != 0:
+            raise RuntimeError(f"solve of request {i}: status {st}")
+        results[i] = res
+        lat[i] = time.perf_counter() - t0 - arr[i]
+    return results, lat, time.perf_counter() - t0
+
+
+def same_result(a, b):
+    import numpy as np
+    if a.counts() != b.counts():
+        return False
+    n = a.summary.npaths
+    return all(np.array_equal(a.p[f][:n], b.p[f][:n])
+               for f in ("z", "status", "winding", "solution_index",
+                         "singular_index"))
+
+
+def class_kernel(c):
+    """Kernel name of an engine class (what rocprofv3 lists)."""
+    return "phc::track_kernel<%d, false, %d>" % (int(c["G"]), int(c["mapping"]))
+
+
+def stats_delta(after, before):
+    return [{k: a[k] - (b[k] if k in ("launches", "kernel_ms", "units",
+                                       "unit_slots") else 0)
+             for k in a} for a, b in zip(after, before)]
+
+
+def serving_section(ctx, H, args, out):
+    import numpy as np
+    from polyhomocont_amd import abi, flops, workloads
+    opts = abi.default_options()
+    stream = workloads.serving_stream(args.serving_requests)
+    paths = sum(u.bezout for u in stream)
+    log(f"serving stream: {len(stream)} requests, {paths} paths")
+    eng = Engine(ctx, H)
+    for w in range(args.warmup):
+        eng.serve(stream, opts)
+        log(f"warmup pass {w + 1}/{args.warmup}")
+    import torch
+    torch.cuda.synchronize()
+    before = eng.srv.stats()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        res, lat, cls, recs, _ = eng.serve(stream, opts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    after = eng.srv.stats()
+    log(f"timed {args.steps} passes: {wall / args.steps:.3f} s/pass")
+    value = paths * args.steps / wall
+    # roofline of the dominant class over the timed passes
+    delta = stats_delta(after, before)
+    dom = int(np.argmax([d["kernel_ms"] for d in delta]))
+    fl = np.zeros(len(delta))
+    for i, (packed, rec) in enumerate(recs):
+        fl[cls[i]] += flops.path_flops(packed, rec.nlin[:rec.npaths],
+                                       rec.nbe[:rec.npaths]).sum()
+    d = delta[dom]
+    class_flops = fl[dom] * args.steps
+    achieved = class_flops / (d["kernel_ms"] / 1e3) / 1e12
+    per_launch_ms = d["kernel_ms"] / max(d["launches"], 1)
+    out["value"] = round(value, 3)
+    out["ms_per_step"] = round(wall / args.steps * 1e3, 4)
+    out["config"] = {
+        "workload": f"serving stream: {len(stream)} requests drawn at random, "
+                    "half random sparse (n 3..8, degree 2..4, 5..40 terms), "
+                    "half dense (n 2..7, degree 2..3), 8..4096 paths each, "
+                    f"{paths} paths; continuous-batching engine, host "
+                    "classification included",
+        "requests": len(stream), "paths": paths,
+        "units_per_launch": UNITS_PER_LAUNCH, "resident_path_slots": 16384,
+        "host_pool_threads": 4}
+    out["roofline"] = {
+        "bound": "fp64-valu",
+        "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
+        "traffic": load_traffic("serving"),
+        "kernel": class_kernel(after[dom]),
+        "flops_per_launch": int(class_flops / max(d["launches"], 1)),
+        "avg_launch_ms": round(per_launch_ms, 4),
+        "launches": int(d["launches"]),
+        "note": "FP64 vector ALU work (no MFMA); canonical flops of the "
+                "paths the class tracked over the kernel time of its "
+                "launches"}
+    out["serving_classes"] = [
+        {"kernel": class_kernel(x), "term_values": int(x["vmax"]),
+         "kernel_ms": round(x["kernel_ms"], 2),
+         "launches": int(x["launches"]), "units": int(x["units"]),
+         "tflops": round(fl[i] * args.steps / max(x["kernel_ms"], 1e-9) / 1e9, 4),
+         "slot_time_utilisation": round(
+             x["units"] / max(x["slots"] * x["launches"] * UNITS_PER_LAUNCH, 1), 4),
+         "lane_occupancy": round(x["units"] / x["unit_slots"], 4)
+         if x["unit_slots"] else None}
+        for i, x in enumerate(delta)]
+    # plain per-solve path, saturation
+    plain_res, _, plain_s = serve_plain(H, stream, opts)
+    log(f"plain per-solve path: {plain_s:.2f} s")
+    identical = sum(same_result(a, b) for a, b in zip(res, plain_res))
+    serving = {
+        "engine_paths_per_s": round(value, 1),
+        "plain_paths_per_s": round(paths / plain_s, 1),
+        "speedup_vs_plain": round(value / (paths / plain_s), 3),
+        "results_identical_to_plain": f"{identical}/{len(stream)}",
+    }
+    # latency at 70% of the plain saturation rate, then a sweep
+    plain_rate = len(stream) / plain_s
+    rng = np.random.default_rng(105)
+    gaps = rng.exponential(1.0, len(stream))
+
+    def arrivals(rate):
+        return list(np.cumsum(gaps) / rate)
+
+    lam = 0.7 * plain_rate
+    _, lat_e, _, _, _ = eng.serve(stream, opts, arrivals(lam))
+    _, lat_p, _ = serve_plain(H, stream, opts, arrivals(lam))
+    serving["latency_at_fixed_rate"] = {
+        "requests_per_s": round(lam, 3), "fraction_of_plain_saturation": 0.7,
+        "engine_ms": percentiles(lat_e), "plain_ms": percentiles(lat_p)}
+    log(f"latency at {lam:.1f} req/s: engine p50/p99 "
+        f"{serving['latency_at_fixed_rate']['engine_ms']['p50']}/"
+        f"{serving['latency_at_fixed_rate']['engine_ms']['p99']} ms, plain "
+        f"{serving['latency_at_fixed_rate']['plain_ms']['p50']}/"
+        f"{serving['latency_at_fixed_rate']['plain_ms']['p99']} ms")
+    sweep = []
+    if not args.no_sweep:
+        # the plain path saturates at 1.0; the engine is swept beyond it
+        for f in (0.5, 1.0, 2.0, 4.0):
+            rate = f * plain_rate
+            _, le, _, _, _ = eng.serve(stream, opts, arrivals(rate))
+            row = {"fraction_of_plain_saturation": f,
+                   "requests_per_s": round(rate, 3),
+                   "engine_ms": percentiles(le)}
+            sweep.append(row)
+            log(f"sweep {f}: engine p50/p99 {row['engine_ms']['p50']}/"
+                f"{row['engine_ms']['p99']} ms")
+    serving["latency_sweep"] = sweep
+    serving["latency_includes"] = (
+        "arrival -> classified result on the host: constants and start "
+        "points, tracking, classification, refinement, deduplication")
+    eng.close()
+    return stream, res, serving
+
+
+def cpu_stream_baseline(stream, engine_results, budget_s):
+    """The compiled reference `solve` on one host core over the first
+    requests of the stream, within about budget_s seconds; the results are
+    compared with the engine's."""
     import numpy as np
     import oracle
+    from polyhomocont_amd import abi
     ref = oracle.ref()
     kind = "reference"
     if ref is None:
         ref = oracle.port()
         kind = "port"
-    # calibrate on a few paths, then time a sample that fills the budget
+    opts = abi.default_options()
     t0 = time.perf_counter()
-    ref.endpoints(usys, opts, 0, 4)
-    per = max((time.perf_counter() - t0) / 4, 1e-4)
-    count = int(min(npaths, max(8, budget_s / per)))
-    t0 = time.perf_counter()
-    rec = ref.endpoints(usys, opts, 0, count)
+    done, paths, same, maxerr = 0, 0, 0, 0.0
+    for i, usys in enumerate(stream):
+        if time.perf_counter() - t0 > budget_s and done:
+            break
+        st, want = ref.solve(usys, opts)
+        got = engine_results[i]
+        n = want.summary.npaths
+        same += int(got.counts() == want.counts() and
+                    np.array_equal(got.p["status"][:n], want.p["status"][:n]))
+        s = want.summary.nsolutions
+        if s:
+            scale = np.maximum(np.abs(want.solutions[:s]).max(axis=1), 1.0)
+            err = (np.abs(got.solutions[:s] - want.solutions[:s]).max(axis=1)
+                   / scale).max()
+            maxerr = max(maxerr, float(err))
+        done += 1
+        paths += usys.bezout
     dt = time.perf_counter() - t0
-    return dict(value=count / dt, unit="paths/s", cores=1, kind=kind,
-                sample=f"paths 1..{count} of the same solve, serial "
-                       f"(1 host core, {dt:.1f} s)"), rec, count
+    return ({"value": round(paths / dt, 2), "unit": "paths/s", "cores": 1,
+             "kind": kind,
+             "sample": f"reference solve of requests 1..{done} of the same "
+                       f"stream ({paths} paths), serial, 1 host core, "
+                       f"{dt:.1f} s"},
+            {"requests_compared": done, "counts_and_status_equal": same,
+             "max_rel_solution_err": maxerr})
 
 
-def _cpu_chunk(args):
-    """One slice of the all-core CPU baseline (runs in a worker process)."""
-    name, first, count = args
+def _ref_solve_request(args):
+    """One request of the stream solved by the reference (worker process)."""
+    count, i = args
     import oracle
     from polyhomocont_amd import abi, workloads
     ref = oracle.ref() or oracle.port()
-    ref.endpoints(workloads.config_system(name), abi.default_options(),
-                  first, count)
-    return count
+    usys = workloads.serving_stream(count)[i]
+    st, _ = ref.solve(usys, abi.default_options())
+    return usys.bezout
 
 
-def cpu_all_cores(name, npaths, per_path_s, budget_s):
-    """The same reference tracker on every host core this process may use
-    (at most 16), paths handed out in chunks of 8 to worker processes.
-    Worker start-up is excluded: the pool is warmed before timing."""
+def cpu_stream_all_cores(count, first_requests, budget_s):
+    """The reference `solve` over the first requests of the stream, one
+    request per task, on every host core this process may use (at most
+    16). Worker start-up is excluded: the pool is warmed before timing."""
     import multiprocessing as mp
     import oracle
     kind = "reference" if oracle.ref() is not None else "port"
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
-    count = int(min(npaths, max(8 * cores, cores * budget_s / per_path_s)))
-    chunks = [(name, f, min(8, count - f)) for f in range(0, count, 8)]
+    m = max(cores, first_requests * cores)
     with mp.get_context("spawn").Pool(cores) as pool:
-        pool.map(_cpu_chunk, [(name, 0, 1)] * cores, chunksize=1)
+        pool.map(_ref_solve_request, [(count, 0)] * cores, chunksize=1)
         t0 = time.perf_counter()
-        done = sum(pool.imap_unordered(_cpu_chunk, chunks))
+        paths = sum(pool.imap_unordered(_ref_solve_request,
+                                        [(count, i) for i in range(min(m, count))]))
         dt = time.perf_counter() - t0
-    return dict(value=done / dt, unit="paths/s", cores=cores, kind=kind,
-                sample=f"paths 1..{count} of the same solve over {cores} "
-                       f"worker processes ({dt:.1f} s)")
+    return {"value": round(paths / dt, 2), "unit": "paths/s", "cores": cores,
+            "kind": kind,
+            "sample": f"reference solve of requests 1..{min(m, count)} of the "
+                      f"same stream ({paths} paths) over {cores} worker "
+                      f"processes ({dt:.1f} s)"}
 
 
-def percentiles(xs):
+# ---------------------------------------------------------------------- C2
+def c2_section(ctx, H, args):
     import numpy as np
-    xs = np.asarray(xs) * 1e3
-    return {"p50": round(float(np.percentile(xs, 50)), 3),
-            "p99": round(float(np.percentile(xs, 99)), 3),
-            "mean": round(float(xs.mean()), 3)}
-
-
-def serving_benchmark(ctx, host_lib, count, units_per_launch=64):
-    """The mixed request stream served by (a) the plain per-solve tracker,
-    one request at a time, and (b) the continuous-batching engine:
-    saturation throughput with every request queued at t = 0, then
-    per-request latency (arrival -> endpoint records on the host) under
-    Poisson arrivals at 70% of the plain tracker's saturation rate."""
-    import numpy as np
-    from polyhomocont_amd import abi, device, workloads
-
-    systems = workloads.serving_stream(count)
-    reqs = []
-    for usys in systems:
-        opts = abi.default_options()
-        g, b, a, ns = host_lib.homotopy(usys, opts.seed)
-        reqs.append((host_lib.pack(usys), abi.HomotopyData(g, b, a), opts,
-                     host_lib.start_points(usys, opts.seed)))
-    paths = sum(len(r[3]) for r in reqs)
-    log(f"serving: {len(reqs)} requests, {paths} paths prepared")
-    # warm both paths (code objects, device buffers)
-    ctx.track_paths(*reqs[0])
-    srv = device.Server(ctx, max_resident_paths=16384,
-                        hbm_budget_bytes=2 << 30,
-                        units_per_launch=units_per_launch)
-    t, _ = srv.submit(*reqs[0])
-    srv.wait(t)
-
+    from polyhomocont_amd import abi, device, flops, workloads
+    usys = workloads.config_system("c2")
+    opts = abi.default_options()
+    gamma, b, patch, npaths = H.homotopy(usys, opts.seed)
+    start = H.start_points(usys, opts.seed)
+    packed = H.pack(usys)
+    hom = abi.HomotopyData(gamma, b, patch)
+    tune = ctx.autotune(packed, hom, opts, start)
+    st = device.Staged(ctx, packed, hom, opts, start)
+    for _ in range(2):
+        st.run()
+        st.sync()
+    steps = 10
+    kernel_ms = []
     t0 = time.perf_counter()
-    plain_out = [ctx.track_paths(*r) for r in reqs]
-    plain_s = time.perf_counter() - t0
-    log(f"serving: plain tracker saturation {plain_s:.2f} s")
-
-    t0 = time.perf_counter()
-    tickets = [srv.submit(*r)[0] for r in reqs]
-    for t in tickets:
-        srv.wait(t)
-    serve_s = time.perf_counter() - t0
-    log(f"serving: engine saturation {serve_s:.2f} s")
-    same = all(np.array_equal(srv._keep[t][4].z, p.z) and
-               np.array_equal(srv._keep[t][4].status, p.status)
-               for t, p in zip(tickets, plain_out))
-    for t in tickets:
-        srv.release(t)
-
-    lam = 0.7 * len(reqs) / plain_s
-    rng = np.random.default_rng(105)
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
-
-    # plain tracker: one FIFO server
-    lat_plain = []
-    t0 = time.perf_counter()
-    for r, a in zip(reqs, arrivals):
-        now = time.perf_counter() - t0
-        if now < a:
-            time.sleep(a - now)
-        ctx.track_paths(*r)
-        lat_plain.append(time.perf_counter() - t0 - a)
-    log("serving: plain tracker latency run done")
-
-    # serving engine: submit on arrival, progress continuously
-    lat_serve = [None] * len(reqs)
-    pending = {}
-    nxt = 0
-    t0 = time.perf_counter()
-    while nxt < len(reqs) or pending:
-        now = time.perf_counter() - t0
-        if now > arrivals[-1] + 120.0:
-            raise RuntimeError("serving latency run did not finish")
-        while nxt < len(reqs) and arrivals[nxt] <= now:
-            pending[srv.submit(*reqs[nxt])[0]] = nxt
-            nxt += 1
-        if not pending:
-            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
-            continue
-        srv.progress()
-        for t in [t for t in pending if srv.done(t)]:
-            i = pending.pop(t)
-            lat_serve[i] = time.perf_counter() - t0 - arrivals[i]
-            srv.release(t)
-    srv.close()
-    log("serving: engine latency run done")
-    fe_s, fe_same, fe_bytes = frontend_saturation(
-        ctx, host_lib, systems, plain_out, units_per_launch)
-    log(f"serving: front end saturation {fe_s:.2f} s")
-    return {
-        "workload": f"{len(reqs)} requests, alternately random sparse "
-                    "(n 3..8, degree 2..4, 5..40 terms) and dense (n 2..7, "
-                    "degree 2..3), 8..4096 paths each",
-        "paths": paths,
-        "units_per_launch": units_per_launch,
-        "plain_paths_per_s": round(paths / plain_s, 1),
-        "serving_paths_per_s": round(paths / serve_s, 1),
-        "serving_speedup": round(plain_s / serve_s, 2),
-        "records_identical": bool(same),
-        "qps": round(lam, 3),
-        "latency_ms": {"serving": percentiles(lat_serve),
-                       "plain": percentiles(lat_plain)},
-        "front_end": {
-            "paths_per_s": round(paths / fe_s, 1),
-            "records_identical": bool(fe_same),
-            "request_bytes": fe_bytes,
-            "includes": "request frames over TCP on 127.0.0.1, decoding, "
-                        "building each system and drawing its start points "
-                        "on the front end's thread, the engine, reply "
-                        "frames"},
+    for _ in range(steps):
+        st.run()
+        kernel_ms.append(st.elapsed_ms())
+    st.sync()
+    wall = time.perf_counter() - t0
+    rec = st.fetch()
+    info = st.launch_info()
+    c = ctx.counters()
+    st.close()
+    fl = int(flops.path_flops(packed, rec.nlin[:npaths], rec.nbe[:npaths]).sum())
+    avg_s = float(np.mean(kernel_ms)) / 1e3
+    achieved = fl / avg_s / 1e12
+    traffic = load_traffic("c2")
+    out = {
+        "workload": "dense n=7, degree 3, 120 monomials/equation, 2187 paths, "
+                    "plain tracker, inputs resident in HBM",
+        "paths_per_s": round(npaths * steps / wall, 1),
+        "ms_per_solve": round(wall / steps * 1e3, 3),
+        "launch": info, "autotune": tune,
+        "roofline": {"bound": "fp64-valu", "achieved": round(achieved, 4),
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
+                     "traffic": traffic,
+                     "kernel": "phc::track_kernel<%d, %s, %d>" % (
+                         info["group"], "true" if info["staged"] else "false",
+                         {"row": 0, "split": 1, "grid": 2}[info["mapping"]]),
+                     "flops_per_launch": fl,
+                     "avg_launch_ms": round(avg_s * 1e3, 4)},
+        "lane_occupancy": round(c["occupancy"], 4),
+        "slot_time_utilisation": round(c["slot_utilisation"], 4),
+        "resident_path_slots": c["slots"],
+        "longest_slot_units": c["max_slot_units"],
+        "units": c["units"],
     }
-
-
-def frontend_saturation(ctx, host_lib, systems, plain_out, units_per_launch):
-    """All requests sent at once as frames to the front end (own thread,
-    own engine) by one client; returns (seconds until the last reply,
-    records equal to the plain tracker's, request bytes)."""
-    import threading
-    import numpy as np
-    from polyhomocont_amd import abi, device, frontend, wire
-
-    frames = [wire.encode_request(u, abi.default_options(), i)
-              for i, u in enumerate(systems)]
-    srv = device.Server(ctx, max_resident_paths=16384,
-                        hbm_budget_bytes=2 << 30,
-                        units_per_launch=units_per_launch)
-    fe = frontend.FrontEnd(srv, host_lib)
-    stop = threading.Event()
-    th = threading.Thread(target=fe.serve, args=(stop,), daemon=True)
-    th.start()
-    got = {}
-    try:
-        cl = frontend.Client(fe.address)
+    if not args.no_cpu:
+        import oracle
+        ref = oracle.ref() or oracle.port()
         t0 = time.perf_counter()
-        for f in frames:
-            wire.send_frame(cl.sock, f)
-        for _ in frames:
-            rid, rec, err, msg = cl.receive()
-            if err:
-                raise RuntimeError(f"front end rejected request {rid}: {msg}")
-            got[rid] = rec
+        ref.endpoints(usys, opts, 0, 4)
+        per = max((time.perf_counter() - t0) / 4, 1e-4)
+        cnt = int(min(npaths, max(8, args.cpu_seconds / 2 / per)))
+        t0 = time.perf_counter()
+        want = ref.endpoints(usys, opts, 0, cnt)
         dt = time.perf_counter() - t0
-        cl.close()
-    finally:
-        stop.set()
-        th.join(timeout=60)
-        fe.close()
-        srv.close()
-    same = all(np.array_equal(got[i].z[:got[i].npaths], p.z[:got[i].npaths])
-               and np.array_equal(got[i].status[:got[i].npaths],
-                                  p.status[:got[i].npaths])
-               for i, p in enumerate(plain_out))
-    return dt, same, sum(len(f) for f in frames)
+        out["cpu_baseline"] = {"value": round(cnt / dt, 2), "unit": "paths/s",
+                               "cores": 1,
+                               "sample": f"paths 1..{cnt}, serial ({dt:.1f} s)"}
+        out["parity"] = {
+            "paths_compared": cnt,
+            "bit_identical": bool(all(np.array_equal(getattr(rec, f)[:cnt],
+                                                     getattr(want, f)[:cnt])
+                                      for f in ("z", "status", "winding",
+                                                "eg_status", "naccepted",
+                                                "nrejected")))}
+    return out
 
 
-def batch_benchmark(ctx, host_lib, count):
-    """The batched multi-system launch: the first `count` of the seeded
-    random sparse systems (n 3..8, degree 2..4, 5..40 terms per equation),
-    all their paths in one persistent launch per lanes class."""
+# ------------------------------------------------------------------- batch
+def batch_section(ctx, H, count):
     from polyhomocont_amd import abi, workloads
     items = []
     for usys in workloads.batch_systems(count):
         opts = abi.default_options()
-        g, b, a, ns = host_lib.homotopy(usys, opts.seed)
-        items.append((host_lib.pack(usys), abi.HomotopyData(g, b, a), opts,
-                      host_lib.start_points(usys, opts.seed)))
+        g, b, a, ns = H.homotopy(usys, opts.seed)
+        items.append((H.pack(usys), abi.HomotopyData(g, b, a), opts,
+                      H.start_points(usys, opts.seed)))
     paths = sum(len(it[3]) for it in items)
     log(f"batch: {count} systems, {paths} paths")
     t0 = time.perf_counter()
     outs = ctx.track_batch(items)
     dt = time.perf_counter() - t0
-    occ = ctx.counters()["occupancy"]
-    ok = sum(int((o.status[:len(it[3])] == 0).sum()) for o, it in zip(outs, items))
+    c = ctx.counters()
     log(f"batch: {dt:.2f} s")
     return {"workload": f"{count} random sparse systems (n 3..8, degree 2..4, "
                         "5..40 terms per equation), one launch per lanes class",
             "systems": count, "paths": paths,
             "paths_per_s": round(paths / dt, 1), "seconds": round(dt, 3),
-            "paths_success": ok,
-            "lane_occupancy_after_divergence": round(occ, 4),
-            "includes": "host packing excluded; upload, kernels, download included"}
+            "units": c["units"],
+            "lane_occupancy_after_divergence": round(c["occupancy"], 4),
+            "includes": "host packing excluded; upload, kernels, download "
+                        "included"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--steps", type=int, default=3,
+                    help="timed passes over the serving stream")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving-requests", type=int, default=48)
-    ap.add_argument("--serving-units", type=int, default=64,
-                    help="linearizations per path slot per engine launch")
-    ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="systems of the batched multi-system launch (0: skip)")
     args = ap.parse_args()
@@ -311,125 +503,38 @@ def main():
               "order for classification)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    # torch first: it loads its own HIP runtime, which the library's must
+    # then share (a runtime loaded before torch's leaves torch without GPUs)
     import torch
-    from polyhomocont_amd import abi, device, flops, host, workloads
-
-    usys = workloads.config_system(args.workload)
-    opts = abi.default_options()
+    torch.cuda.init()
+    from polyhomocont_amd import device, host
     H = host.library()
-    gamma, b, patch, npaths = H.homotopy(usys, opts.seed)
-    start = H.start_points(usys, opts.seed)
-    packed = H.pack(usys)
-    hom = abi.HomotopyData(gamma, b, patch)
-
     ctx = device.Context(0)
-    tune = None
-    if not args.no_autotune:
-        tune = ctx.autotune(packed, hom, opts, start)
-        log(f"autotune: {tune}")
-    st = device.Staged(ctx, packed, hom, opts, start)
-    log(f"staged {args.workload}: {npaths} paths, launch {st.launch_info()}")
-    for _ in range(args.warmup):
-        st.run()
-        st.sync()
-
-    kernel_ms = []
-    torch.cuda.synchronize()
-    st.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st.run()
-        kernel_ms.append(st.elapsed_ms())
-    st.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    log(f"timed {args.steps} steps: {(t1 - t0) / args.steps * 1e3:.2f} ms/step")
-    wall = t1 - t0
-    rec = st.fetch()
-    info = st.launch_info()
-    occ = ctx.counters()["occupancy"]
-
-    value = npaths * args.steps / wall
-    flops_launch = int(flops.path_flops(packed, rec.nlin[:npaths],
-                                        rec.nbe[:npaths]).sum())
-    avg_s = float(np.mean(kernel_ms)) / 1e3
-    achieved = flops_launch / avg_s / 1e12
-    traffic = load_traffic(args.workload)
-
     out = {
         "metric": "paths/sec at endpoint err <=1e-10 vs CPU ref; p50/p99 "
                   "solve latency @ fixed QPS",
-        "value": round(value, 3),
-        "unit": "paths/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        "value": None, "unit": "paths/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c128 (fp64)",
         "data": "synthetic (seeded complex-Gaussian coefficients)",
-        "config": {"workload": f"{args.workload}: dense n=7, degree 3, 120 "
-                               "monomials/equation, 2187 paths, plain per-solve "
-                               "GPU tracker" if args.workload == "c2" else args.workload,
-                   "npaths": npaths, "lanes_per_path": info["lanes_per_path"],
-                   "mapping": info["mapping"],
-                   "autotuned": tune is not None,
-                   "workgroups": info["workgroups"],
-                   "threads_per_workgroup": info["threads"]},
-        "roofline": {
-            "bound": "mfma",
-            "achieved": round(achieved, 4),
-            "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_TFLOPS, 6),
-            "traffic": traffic,
-            "kernel": "phc::track_kernel<%d, %s, %d> (%s mapping)" % (
-                info["group"], "true" if info["staged"] else "false",
-                {"row": 0, "split": 1, "grid": 2}[info["mapping"]],
-                info["mapping"]),
-            "flops_per_launch": flops_launch,
-            "avg_launch_ms": round(avg_s * 1e3, 4),
-            "hbm_gbps": round(traffic / avg_s / 1e9, 3) if traffic else None,
-            "note": "FP64 vector ALU work (no MFMA); peak is the MI355X "
-                    "FP64 figure",
-        },
-        "path_divergence": {
-            "paths_per_wavefront": 64 // info["lanes_per_path"],
-            "lane_occupancy": round(occ, 4),
-            "loss": round(1.0 - occ, 4),
-            "definition": "1 - units run / (passes of each wavefront's "
-                          "tracking loop that ran a unit x paths per "
-                          "wavefront), device counters of the last timed launch"},
-        "work": {"linearizations": int(rec.nlin[:npaths].sum()),
-                 "backward_errors": int(rec.nbe[:npaths].sum()),
-                 "steps_accepted": int(rec.naccepted[:npaths].sum()),
-                 "steps_rejected": int(rec.nrejected[:npaths].sum())},
     }
-
+    stream, results, serving = serving_section(ctx, H, args, out)
+    out["serving"] = serving
     if not args.no_cpu:
-        log("cpu baseline")
-        cb, want, cnt = cpu_baseline(usys, opts, npaths, args.cpu_seconds)
+        log("cpu baseline (reference solve on the stream)")
+        cb, par = cpu_stream_baseline(stream, results, args.cpu_seconds)
         out["cpu_baseline"] = cb
-        log("cpu baseline, all cores")
-        out["cpu_baseline_all_cores"] = cpu_all_cores(
-            args.workload, npaths, 1.0 / cb["value"], args.cpu_seconds / 2)
-        same = all(np.array_equal(getattr(rec, f)[:cnt], getattr(want, f)[:cnt])
-                   for f in ("status", "winding", "eg_status"))
-        scale = np.maximum(np.abs(want.z[:cnt]).max(axis=1), 1e-300)
-        err = float((np.abs(rec.z[:cnt] - want.z[:cnt]).max(axis=1) / scale).max())
-        bit = float((rec.z[:cnt] == want.z[:cnt]).all(axis=1).mean())
-        out["parity"] = {"paths_compared": cnt, "max_rel_endpoint_err": err,
-                         "classification_equal": bool(same),
-                         "bit_identical_fraction": bit}
-        out["gpu_vs_cpu_core"] = round(value / cb["value"], 1)
+        out["cpu_baseline_all_cores"] = cpu_stream_all_cores(
+            len(stream), par["requests_compared"], args.cpu_seconds)
+        out["parity"] = par
+        out["gpu_vs_cpu_core"] = round(out["value"] / cb["value"], 1)
+    if not args.no_c2:
+        log("c2 plain tracker")
+        out["c2"] = c2_section(ctx, H, args)
     if args.batch_systems > 0:
-        out["batch"] = batch_benchmark(ctx, H, args.batch_systems)
-    if not args.no_serving:
-        out["serving"] = serving_benchmark(ctx, H, args.serving_requests,
-                                           args.serving_units)
+        out["batch"] = batch_section(ctx, H, args.batch_systems)
+    ctx.close()
     print(json.dumps(out), flush=True)
 
 

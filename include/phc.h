@@ -186,12 +186,15 @@ int phc_ctx_destroy(phc_ctx *ctx);
 int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
 
 /* Device counters of the last tracking call on this context:
+ * out[0] resident path slots of the launch (workgroups x paths each);
  * out[1] linearization units executed (all paths); out[2] unit slots
  * offered: over all wavefronts, the passes of the tracking loop in which at
  * least one of the wavefront's paths ran a unit, times the paths a
  * wavefront holds. out[1] / out[2] is the lane occupancy left after path
- * divergence (1 when a wavefront holds one path). out[0], out[3] are
- * reserved (0). */
+ * divergence (1 when a wavefront holds one path). out[3] the most units
+ * one path slot ran: out[1] / (out[0] x out[3]) is the slot-time
+ * utilisation (the share of the slots' time spent on units while the
+ * busiest slot works, counting units as equal). */
 int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
 
 /* Lane mapping of later tracking calls. 0 chooses per launch: an
@@ -343,6 +346,16 @@ int phc_server_wait(phc_server *srv, uint64_t ticket);
  * are on the device; PHC_ERR_INVALID_INPUT for an unknown ticket. A
  * long-running server calls it once per request, after poll/wait. */
 int phc_server_release(phc_server *srv, uint64_t ticket);
+/* Per lanes class of the engine (returns their number, at most 8): out[8c
+ * + 0] lanes per group G, [1] mapping (0 row, 2 grid), [2] kernel launches
+ * so far, [3] their kernel time in ms (HIP events around each round on the
+ * class's stream), [4] linearization units run, [5] unit slots offered
+ * (see phc_ctx_counters), [6] resident path slots, [7] term values a grid
+ * path holds (0 for a row class). */
+int phc_server_stats(phc_server *srv, double *out);
+/* Lanes class (index into phc_server_stats) that serves a queued or
+ * admitted ticket. */
+int phc_server_ticket_class(phc_server *srv, uint64_t ticket);
 /* Most paths one request may have (the smallest class's ring capacity). */
 int64_t phc_server_max_request_paths(const phc_server *srv);
 /* counts[0] queued, [1] admitted, [2] finished and not yet released. */

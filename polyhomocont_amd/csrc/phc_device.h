@@ -104,7 +104,10 @@ enum : int32_t { RET_MAIN = 0, RET_RADIAL = 1, RET_ARC = 2 };
 // Caller of loops / record inside cauchy_endgame.
 enum : int32_t { CTX_MAIN = 0, CTX_FALLBACK = 1 };
 
-struct PathState {
+// Scalars of a path's state machine: one copy per path, in LDS, read and
+// written by every lane of the path (all lanes hold the same values, so
+// their simultaneous writes store the same bits).
+struct PathScalars {
   int32_t pc, req, path, ret;
   // track_segment (src/tracker.f90:207-221)
   int32_t sk, sj, stage, nit, nsucc, sacc, srej, sstatus, o_grow;
@@ -117,12 +120,28 @@ struct PathState {
   int32_t lc, lj, lmax, lctx, nsamp, rctx, lok;
   double r, rnew, best_resid, dprev, cest, cest_prev, eg_err, eg_resid,
       seg_init, rec_err, spacing, floor;
-  // this lane's component of the path's vectors
-  cx zt;    // point tracked by the current segment
-  cx zq;    // evaluation point / Newton iterate
-  cx acc;   // RK4 accumulator k1 + 2 k2 + 2 k3 + k4
-  cx zeg;   // end game point on the real axis
-  cx est, prev, zold, resz, zprev, zsum;
+};
+
+// The path's N-vectors, [vector][component] in LDS after the scalars; lane
+// j of every lane group of the path works on component j:
+//   ZT point tracked by the current segment, ZQ evaluation point / Newton
+//   iterate, ACC RK4 accumulator k1 + 2 k2 + 2 k3 + k4, ZEG end game point
+//   on the real axis, then the end game's estimates and loop sums.
+enum : int { V_ZT, V_ZQ, V_ACC, V_ZEG, V_EST, V_PREV, V_ZOLD, V_RESZ,
+             V_ZPREV, V_ZSUM, NVEC };
+
+// LDS of one path's state: the scalars (16-byte aligned) and NVEC x G
+// components.
+PHC_HD constexpr size_t state_lds_bytes(int G) {
+  return ((sizeof(PathScalars) + 15) & ~size_t(15)) +
+         (size_t)NVEC * G * sizeof(cx);
+}
+
+// Saved state of one lane of a path between launches of the resumable
+// kernel (serving engine): the scalars and the lane's components.
+struct PathState {
+  PathScalars s;
+  cx v[NVEC];
 };
 
 // A queued path of the serving engine: request slot and path index.
@@ -148,7 +167,8 @@ struct Launch {
   int32_t resume;           // load saved states at entry
   int32_t max_units;        // units per group per launch, 0 = unlimited
   unsigned int* active;     // groups still holding a path at exit
-  unsigned long long* counters;  // [4]: reserved, units, unit slots, reserved
+  unsigned long long* counters;  // [4]: reserved, units, unit slots, most
+                                 // units run by one path slot
 };
 
 // ---------------------------------------------------------------- group ops
@@ -231,7 +251,7 @@ PHC_D void wave_lds_fence() {
 // tangent needs it). Lines are real (all lines the
 // tracker uses are); arcs evaluate cos and sin of theta0 + s dtheta as
 // glibc's sincos does.
-PHC_D cx seg_t(const PathState& S, double s) {
+PHC_D cx seg_t(const PathScalars& S, double s) {
   if (S.sk == 0) {
     if (s >= 1.0) return mk(S.sb, 0.0);
     return mk(S.sa + s * (S.sb - S.sa), 0.0);
@@ -605,9 +625,22 @@ enum : int32_t {
 template <int G>
 struct Machine {
   const Req& R;
-  PathState& S;
+  PathScalars& S;
+  cx* const V;  // the path's vectors, [NVEC][G] (LDS)
   const int lane;
   bool writer;  // the replica that stores results (split mode: replica 0)
+
+  // this lane's component of each vector
+  PHC_D cx& zt() const { return V[V_ZT * G + lane]; }
+  PHC_D cx& zq() const { return V[V_ZQ * G + lane]; }
+  PHC_D cx& acc() const { return V[V_ACC * G + lane]; }
+  PHC_D cx& zeg() const { return V[V_ZEG * G + lane]; }
+  PHC_D cx& est() const { return V[V_EST * G + lane]; }
+  PHC_D cx& prev() const { return V[V_PREV * G + lane]; }
+  PHC_D cx& zold() const { return V[V_ZOLD * G + lane]; }
+  PHC_D cx& resz() const { return V[V_RESZ * G + lane]; }
+  PHC_D cx& zprev() const { return V[V_ZPREV * G + lane]; }
+  PHC_D cx& zsum() const { return V[V_ZSUM * G + lane]; }
 
   PHC_D double norm(cx v) const { return gnorm<G>(v, lane, R.N); }
 
@@ -649,14 +682,14 @@ struct Machine {
       S.ds = 1.0 - S.s;
     }
     S.stage = 0;
-    S.zq = S.zt;
+    zq() = zt();
     S.se = S.s;
     S.pc = PC_TAN;
   }
 
   // src/tracker.f90:240-257
   PHC_D void seg_accept() {
-    S.zt = S.zq;
+    zt() = zq();
     S.s = S.sn;
     S.sacc += 1;
     S.nsucc += 1;
@@ -687,26 +720,26 @@ struct Machine {
       return;
     }
     if (S.stage == 0) {
-      S.acc = k;
-      S.zq = cadd(S.zt, cscale(0.5 * S.ds, k));
+      acc() = k;
+      zq() = cadd(zt(), cscale(0.5 * S.ds, k));
       S.se = S.s + 0.5 * S.ds;
       S.stage = 1;
       S.pc = PC_TAN;
     } else if (S.stage == 1) {
-      S.acc = cadd(S.acc, cscale(2.0, k));
-      S.zq = cadd(S.zt, cscale(0.5 * S.ds, k));
+      acc() = cadd(acc(), cscale(2.0, k));
+      zq() = cadd(zt(), cscale(0.5 * S.ds, k));
       S.se = S.s + 0.5 * S.ds;
       S.stage = 2;
       S.pc = PC_TAN;
     } else if (S.stage == 2) {
-      S.acc = cadd(S.acc, cscale(2.0, k));
-      S.zq = cadd(S.zt, cscale(S.ds, k));
+      acc() = cadd(acc(), cscale(2.0, k));
+      zq() = cadd(zt(), cscale(S.ds, k));
       S.se = S.s + S.ds;
       S.stage = 3;
       S.pc = PC_TAN;
     } else {
-      S.acc = cadd(S.acc, k);
-      S.zq = cadd(S.zt, cscale(S.ds / 6.0, S.acc));
+      acc() = cadd(acc(), k);
+      zq() = cadd(zt(), cscale(S.ds / 6.0, acc()));
       S.nit = 1;
       S.dnp = DBL_MAX;
       if (R.maxit < 1)
@@ -723,9 +756,9 @@ struct Machine {
       seg_reject();
       return;
     }
-    S.zq = cadd(S.zq, dz);
+    zq() = cadd(zq(), dz);
     const double dn = norm(dz);
-    const double zn = norm(S.zq);
+    const double zn = norm(zq());
     if (dn <= R.ctol * zn) {
       seg_accept();
       return;
@@ -751,7 +784,7 @@ struct Machine {
 
   // --- endpoint (src/solve.f90:230-251) -------------------------------------
   PHC_D void start_path(const cx zstart) {
-    S.zt = zstart;
+    zt() = zstart;
     S.nacc = S.nrej = S.nlin = S.nbe = 0;
     const double t_end = R.use_endgame ? R.t_start : 0.0;
     seg_begin(0, 1.0, t_end, 0, RET_MAIN, R.step_init, R.step_max,
@@ -764,14 +797,14 @@ struct Machine {
     S.nacc = S.sacc;
     S.nrej = S.srej;
     if (S.sstatus != PHC_PATH_SUCCESS || !R.use_endgame) {
-      finish(S.zt, 0, 0, 0.0, 0.0);
+      finish(zt(), 0, 0, 0.0, 0.0);
       return;
     }
     // cauchy_endgame set-up (src/endgame.f90:160-174)
     S.seg_init = 1.0;  // segopts: step_init = step_max = 1, grow_after = 2
     S.eg_status = PHC_ENDGAME_FAILED;
-    S.resz = S.zt;
-    S.zeg = S.zt;
+    resz() = zt();
+    zeg() = zt();
     S.r = R.t_start;
     S.have_prev = 0;
     S.best_resid = DBL_MAX;
@@ -817,7 +850,7 @@ struct Machine {
       S.pc = X_EG_RETURN;
       return;
     }
-    S.prev = S.est;
+    prev() = est();
     S.have_prev = 1;
     S.pc = X_EG_RADIAL;
   }
@@ -828,18 +861,18 @@ struct Machine {
       S.pc = X_EG_RETURN;
       return;
     }
-    S.zold = S.zeg;
-    S.zt = S.zeg;
+    zold() = zeg();
+    zt() = zeg();
     seg_begin(0, S.r, S.rnew, 0, RET_RADIAL, S.seg_init, 1.0, 2);
   }
 
   PHC_D void eg_after_radial() {
-    S.zeg = S.zt;
+    zeg() = zt();
     S.seg_init = fmin(fmax(S.step, R.step_min), 1.0);
     S.eg_acc += S.sacc;
     S.eg_rej += S.srej;
     if (S.sstatus != PHC_PATH_SUCCESS) {
-      S.zeg = S.zold;
+      zeg() = zold();
       if (!(S.gate || S.r < R.loop_radius)) {
         S.lmax = R.max_winding;
         S.lctx = CTX_FALLBACK;
@@ -850,7 +883,7 @@ struct Machine {
       return;
     }
     S.r = S.rnew;
-    const double d = norm(csub(S.zeg, S.zold));
+    const double d = norm(csub(zeg(), zold()));
     if (!S.gate && S.dprev > 0.0 && d > 0.0) {
       S.cest_prev = S.cest;
       S.cest = -1.0;
@@ -881,8 +914,8 @@ struct Machine {
   PHC_D void record_begin() {
     S.rec_err = DBL_MAX;
     if (S.have_prev)
-      S.rec_err = norm(csub(S.est, S.prev)) / fmax(norm(S.est), DBL_MIN);
-    S.zq = S.est;
+      S.rec_err = norm(csub(est(), prev())) / fmax(norm(est()), DBL_MIN);
+    zq() = est();
     S.pc = PC_BE;
   }
 
@@ -897,7 +930,7 @@ struct Machine {
     }
     if (take) {
       S.eg_status = PHC_ENDGAME_NOT_CONVERGED;
-      S.resz = S.est;
+      resz() = est();
       S.eg_wind = S.lc;
       S.eg_err = err;
       S.eg_resid = resid;
@@ -912,18 +945,18 @@ struct Machine {
     S.nrej += S.eg_rej;
     if (S.eg_status == PHC_ENDGAME_FAILED) {
       S.pstatus = PHC_PATH_ENDGAME_FAILED;
-      finish(S.zeg, 0, 0, 0.0, 0.0);
+      finish(zeg(), 0, 0, 0.0, 0.0);
       return;
     }
     S.pt = 0.0;
-    finish(S.resz, S.eg_wind, S.eg_status, S.eg_err, S.eg_resid);
+    finish(resz(), S.eg_wind, S.eg_status, S.eg_err, S.eg_resid);
   }
 
   // loops (src/endgame.f90:349-376)
   PHC_D void loops_begin() {
-    S.floor = 10.0 * R.ctol * norm(S.zeg);
-    S.zt = S.zeg;
-    S.zsum = mk(0.0, 0.0);
+    S.floor = 10.0 * R.ctol * norm(zeg());
+    zt() = zeg();
+    zsum() = mk(0.0, 0.0);
     S.nsamp = 0;
     S.spacing = DBL_MAX;
     S.lc = 1;
@@ -937,9 +970,9 @@ struct Machine {
   }
 
   PHC_D void loops_arc() {
-    S.zsum = cadd(S.zsum, S.zt);
+    zsum() = cadd(zsum(), zt());
     S.nsamp += 1;
-    S.zprev = S.zt;
+    zprev() = zt();
     seg_begin(1, (double)(S.lj - 1) * R.dtheta, S.r, S.lj - 1, RET_ARC,
               S.seg_init, 1.0, 2);
   }
@@ -953,14 +986,14 @@ struct Machine {
       S.pc = X_LOOPS_END;
       return;
     }
-    S.spacing = fmin(S.spacing, norm(csub(S.zt, S.zprev)));
+    S.spacing = fmin(S.spacing, norm(csub(zt(), zprev())));
     S.lj += 1;
     if (S.lj <= R.nsamples) {
       S.pc = X_LOOPS_ARC;
       return;
     }
-    if (norm(csub(S.zt, S.zeg)) <= fmax(R.closure * S.spacing, S.floor)) {
-      S.est = cdiv(S.zsum, mk((double)S.nsamp, 0.0));
+    if (norm(csub(zt(), zeg())) <= fmax(R.closure * S.spacing, S.floor)) {
+      est() = cdiv(zsum(), mk((double)S.nsamp, 0.0));
       S.lok = 1;
       S.pc = X_LOOPS_END;
       return;
@@ -1046,11 +1079,12 @@ constexpr int lanes_per_path() {
 
 template <int G, int MODE>
 PHC_D size_t unit_lds_bytes(int expmax, int vmax) {
-  return MODE == MODE_GRID
-             ? WaveLds<G>::bytes(expmax, vmax)
-             : (MODE == MODE_SPLIT
-                    ? GroupLds<G>::bytes(expmax) + SplitLds<G>::bytes()
-                    : GroupLds<G>::bytes(expmax));
+  return state_lds_bytes(G) +
+         (MODE == MODE_GRID
+              ? WaveLds<G>::bytes(expmax, vmax)
+              : (MODE == MODE_SPLIT
+                     ? GroupLds<G>::bytes(expmax) + SplitLds<G>::bytes()
+                     : GroupLds<G>::bytes(expmax)));
 }
 
 template <int G, bool STAGED, int MODE>
@@ -1093,8 +1127,12 @@ __global__ void __launch_bounds__(WG)
     }
     __syncthreads();
   }
-  char* mine = lds_raw + base +
-               (size_t)owner * unit_lds_bytes<G, MODE>(L.expmax, L.vmax);
+  // this path's LDS: its state (scalars, then vectors), then the unit's
+  char* const pbase = lds_raw + base +
+                      (size_t)owner * unit_lds_bytes<G, MODE>(L.expmax, L.vmax);
+  PathScalars& S = *(PathScalars*)pbase;
+  cx* const SV = (cx*)(pbase + ((sizeof(PathScalars) + 15) & ~size_t(15)));
+  char* mine = pbase + state_lds_bytes(G);
   GroupLds<G> gl;
   SplitLds<G> sl;
   WaveLds<G> wl;
@@ -1110,12 +1148,16 @@ __global__ void __launch_bounds__(WG)
     if (SPLIT) sl.bind(mine + GroupLds<G>::bytes(L.expmax));
   }
 
-  PathState S;
+  // every lane of the path writes the same scalars and its own components
   if (L.resume) {
-    S = L.slots[(size_t)slot * G + lane];
+    const PathState& g = L.slots[(size_t)slot * G + lane];
+    S = g.s;
+#pragma unroll
+    for (int k = 0; k < NVEC; ++k) SV[k * G + lane] = g.v[k];
   } else {
     S.pc = PC_IDLE;
   }
+  wave_lds_fence();
   int units = 0;
   // passes of this wavefront's loop in which at least one of its paths ran
   // a unit: units / (passes * paths per wavefront) is the lane occupancy
@@ -1168,13 +1210,13 @@ __global__ void __launch_bounds__(WG)
         for (int i = (int)(threadIdx.x % LP); i < L.vmax; i += LP)
           wl.V[i] = mk(0.0, 0.0);
       }
-      Machine<G> M{R, S, lane, writer};
+      Machine<G> M{R, S, SV, lane, writer};
       M.start_path((lane < R.N) ? R.start[(size_t)S.path * R.N + lane]
                                 : mk(0.0, 0.0));
     } else {
       if (L.max_units && units >= L.max_units) break;
       const Req& R = STAGED ? *sreq : L.reqs[S.req];
-      Machine<G> M{R, S, lane, writer};
+      Machine<G> M{R, S, SV, lane, writer};
       cx t = mk(0.0, 0.0), dt = mk(0.0, 0.0);
       if (S.pc != PC_BE) {
         const double sv = (S.pc == PC_TAN) ? S.se : S.sn;
@@ -1187,7 +1229,7 @@ __global__ void __launch_bounds__(WG)
       cx x;
       int ok;
       double be = 0.0;
-      const cx zl = (lane < R.N) ? S.zq : mk(0.0, 0.0);
+      const cx zl = (lane < R.N) ? SV[V_ZQ * G + lane] : mk(0.0, 0.0);
       if (GRID)
         unit_wave<G>(R, STAGED ? swterms : R.wterms, STAGED ? sjobs : R.jobs,
                      lane, rep, S.pc, zl, t, dt, wl, x, ok, be);
@@ -1206,13 +1248,20 @@ __global__ void __launch_bounds__(WG)
         M.on_backward_error(be);
     }
     const Req& R = STAGED ? *sreq : L.reqs[S.req];
-    Machine<G> M{R, S, lane, writer};
+    Machine<G> M{R, S, SV, lane, writer};
     M.run();
   }
-  if (L.slots) L.slots[(size_t)slot * G + lane] = S;
+  if (L.slots) {
+    PathState& g = L.slots[(size_t)slot * G + lane];
+    g.s = S;
+#pragma unroll
+    for (int k = 0; k < NVEC; ++k) g.v[k] = SV[k * G + lane];
+  }
   if (writer && S.pc != PC_IDLE && lane == 0 && L.active) atomicAdd(L.active, 1u);
-  if (writer && lane == 0 && L.counters)
+  if (writer && lane == 0 && L.counters) {
     atomicAdd(&L.counters[1], (unsigned long long)units);
+    atomicMax(&L.counters[3], (unsigned long long)units);
+  }
   if (L.counters) {
     for (int off = 32; off > 0; off >>= 1) passes += __shfl_xor(passes, off);
     if (threadIdx.x % 64 == 0)

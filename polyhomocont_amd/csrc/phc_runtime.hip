@@ -440,8 +440,12 @@ hipError_t launch_g(int G, const Launch& L, bool staged, int blocks,
 // Largest LDS a staged workgroup may use (leaves room for two per CU).
 constexpr size_t STAGED_LDS_LIMIT = 64 * 1024;
 
+// LDS of one path (the device's unit_lds_bytes): its state, then the
+// unit's power table and linear system (row and split mappings, plus the
+// split mapping's exchange buffers) or the grid unit's tables.
 size_t group_lds_bytes(int G, int expmax) {
-  return ((size_t)(expmax + 1) * G + (size_t)G * G + G) * sizeof(cx) +
+  return state_lds_bytes(G) +
+         ((size_t)(expmax + 1) * G + (size_t)G * G + G) * sizeof(cx) +
          (size_t)G * sizeof(int);
 }
 size_t split_lds_bytes(int G) {
@@ -449,9 +453,10 @@ size_t split_lds_bytes(int G) {
   return (R * G * (G + 1) + (size_t)G * G + G) * sizeof(cx);
 }
 size_t grid_lds_bytes(int G, int expmax, int vmax) {
-  return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G +
+  return state_lds_bytes(G) +
+         ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G +
           2 * G) *
-         sizeof(cx);
+             sizeof(cx);
 }
 
 // A prepared multi-request run for one lanes class, with device buffers.
@@ -778,6 +783,8 @@ int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->counters[1] += c[1];  // lane 0 of a group adds
     ctx->counters[2] += c[2];  // lane 0 of a wavefront adds
+    ctx->counters[3] = std::max<unsigned long long>(ctx->counters[3], c[3]);
+    ctx->counters[0] += (unsigned long long)cr.blocks * (cr.threads / cr.lp);
   }
   for (auto& up : st->reqs) {
     HostReq& h = *up;
@@ -1283,9 +1290,10 @@ int phc_track_paths_noalloc(phc_ctx* ctx, const phc_system* sys,
                              s));
   // the host tables above must outlive the copies: the call is synchronous
   HIPCHK(ctx, hipStreamSynchronize(s));
-  ctx->counters[0] = ctx->counters[3] = 0;
+  ctx->counters[0] = (unsigned long long)cr.blocks * (cr.threads / cr.lp);
   ctx->counters[1] = c[1];
   ctx->counters[2] = c[2];
+  ctx->counters[3] = c[3];
   return PHC_OK;
 }
 

@@ -61,6 +61,8 @@ def lib():
         L.phc_server_wait.argtypes = [P, C.c_uint64]
         L.phc_server_release.argtypes = [P, C.c_uint64]
         L.phc_server_tickets.argtypes = [P, P]
+        L.phc_server_stats.argtypes = [P, P]
+        L.phc_server_ticket_class.argtypes = [P, C.c_uint64]
         L.phc_server_max_request_paths.argtypes = [P]
         L.phc_server_max_request_paths.restype = i64
         L.phc_server_destroy.argtypes = [P]
@@ -107,7 +109,11 @@ class Context:
         c = np.zeros(4, np.uint64)
         self.L.phc_ctx_counters(self.h, c.ctypes.data)
         occ = float(c[1]) / float(c[2]) if c[2] else None
-        return dict(units=int(c[1]), unit_slots=int(c[2]), occupancy=occ)
+        util = (float(c[1]) / (float(c[0]) * float(c[3]))
+                if c[0] and c[3] else None)
+        return dict(units=int(c[1]), unit_slots=int(c[2]), occupancy=occ,
+                    slots=int(c[0]), max_slot_units=int(c[3]),
+                    slot_utilisation=util)
 
     def set_mapping(self, mapping=0, wg_threads=0):
         """0: chosen per launch, 1: path per G-lane group, 2: path per
@@ -267,6 +273,23 @@ class Server:
         self.ctx._check(self.ctx.L.phc_server_release(self.h, ticket),
                         "phc_server_release")
         self._keep.pop(ticket, None)
+
+    def stats(self):
+        """Per lanes class: G, mapping, launches, kernel ms, units, unit
+        slots, resident path slots."""
+        out = np.zeros(64)
+        n = self.ctx.L.phc_server_stats(self.h, out.ctypes.data)
+        if n < 0:
+            self.ctx._check(n, "phc_server_stats")
+        keys = ("G", "mapping", "launches", "kernel_ms", "units",
+                "unit_slots", "slots", "vmax")
+        return [dict(zip(keys, out[8 * i:8 * i + 8])) for i in range(n)]
+
+    def ticket_class(self, ticket):
+        c = self.ctx.L.phc_server_ticket_class(self.h, ticket)
+        if c < 0:
+            self.ctx._check(c, "phc_server_ticket_class")
+        return c
 
     def max_request_paths(self):
         """Most paths one request may have."""

@@ -232,15 +232,16 @@ def katsura(n):
     return UserSystemData(N, eqs)
 
 
-def serving_stream(count=64, seed=105, max_paths=4096, min_paths=8):
-    """Requests of the serving benchmark: alternately a system of the
-    random sparse generator and a dense system (n in 2..7, degree 2..3),
-    keeping those with 8..4096 total-degree paths."""
+def serving_stream(count=512, seed=105, max_paths=4096, min_paths=8):
+    """Requests of the serving benchmark: each drawn at random, with equal
+    odds, from the random sparse generator (n 3..8, degree 2..4, 5..40
+    terms) or as a dense system (n in 2..7, degree 2..3, all monomials);
+    those with 8..4096 total-degree paths are kept."""
     g = Lecuyer(seed)
     out = []
     k = 0
     while len(out) < count:
-        if k % 2 == 0:
+        if g.uniform() < 0.5:
             usys = random_sparse(10_000 + seed * 1000 + k)
         else:
             n = g.integer(2, 7)

@@ -19,19 +19,29 @@ def test_more_than_one_gpu_is_refused():
     assert r.stdout == ""
 
 
-def test_cpu_baselines_on_c1():
+def test_cpu_baselines_on_a_short_stream(host):
+    """The CPU baselines of the serving stream report what they measured
+    and compare the reference's results with those given."""
     import bench
     import oracle
     from polyhomocont_amd import abi, workloads
-    if oracle.ref() is None and oracle.port() is None:
-        pytest.skip("no CPU tracker built")
-    usys = workloads.config_system("c1")
-    one, rec, cnt = bench.cpu_baseline(usys, abi.default_options(), 16, 0.5)
-    assert one["cores"] == 1 and one["value"] > 0 and 1 <= cnt <= 16
-    assert one["kind"] in ("reference", "port")
-    allc = bench.cpu_all_cores("c1", 16, 1.0 / one["value"], 0.5)
+    if oracle.ref() is None:
+        pytest.skip("the compiled reference is not built")
+    stream = workloads.serving_stream(4)
+    opts = abi.default_options()
+    host.set_device(-1)          # the host tracker stands in for the GPU
+    try:
+        results = [host.solve(u, opts)[1] for u in stream]
+    finally:
+        host.set_device(0)
+    one, par = bench.cpu_stream_baseline(stream, results, 0.5)
+    assert one["cores"] == 1 and one["value"] > 0
+    assert one["kind"] == "reference"
+    assert par["requests_compared"] >= 1
+    assert par["counts_and_status_equal"] == par["requests_compared"]
+    assert par["max_rel_solution_err"] <= 1e-10
+    allc = bench.cpu_stream_all_cores(4, 1, 0.5)
     assert allc["cores"] >= 1 and allc["value"] > 0
-    assert allc["kind"] == one["kind"]
 
 
 def test_percentiles():
