@@ -63,3 +63,24 @@ def test_server_rejects_oversized_request(ctx, host):
     with pytest.raises(device.DeviceError):
         srv.submit(*big)
     srv.close()
+
+
+def test_released_tickets_leave_nothing_behind(ctx, host):
+    """A long-running engine keeps no record of released tickets; a queued
+    ticket can be cancelled; an unknown ticket is an error; the mix runs on
+    several lanes classes (4-lane grid, 8- and 16-lane grids)."""
+    reqs = mixed_requests(host)
+    srv = device.Server(ctx, max_resident_paths=2048)
+    print("[gpu] server release", flush=True)
+    tickets = [srv.submit(*r)[0] for r in reqs]
+    extra, _ = srv.submit(*reqs[0])
+    srv.release(extra)                      # still queued: cancelled
+    for t in tickets:
+        srv.wait(t)
+        srv.release(t)
+    assert srv.tickets() == dict(queued=0, admitted=0, finished=0)
+    with pytest.raises(device.DeviceError):
+        srv.release(tickets[0])
+    used = {(int(c["G"]), int(c["mapping"])) for c in srv.stats() if c["units"]}
+    assert {(4, 2), (8, 2), (16, 2)} <= used, used
+    srv.close()
