@@ -334,8 +334,11 @@ int phc_server_submit(phc_server *srv, const phc_system *sys,
                       const phc_homotopy *hom, const phc_solve_options *opts,
                       int64_t npaths, const double *start, phc_endpoints *out,
                       uint64_t *ticket);
-/* Advances the device work (admission, launches, retirement) once without
- * blocking; returns the number of requests completed by this call. */
+/* One step of the engine: waits for the round of launches in flight (if
+ * any), retires the requests it finished (records copied to their `out`),
+ * admits queued requests that fit, launches the next round and returns
+ * while it runs, so the caller's host work until the next call overlaps
+ * the GPU. Returns the number of requests completed by this call. */
 int phc_server_progress(phc_server *srv);
 /* PHC_OK when the request's records are in its `out`, PHC_ERR_PENDING if
  * not yet; phc_server_wait blocks (driving progress) until done. */

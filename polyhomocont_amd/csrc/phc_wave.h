@@ -534,6 +534,109 @@ PHC_D int lu_slots(int N, cx (&a)[GridShape<G>::E],
   return ok;
 }
 
+// LU steps of a grid with one column per lane (E = 1: G = 4 and G = 8;
+// src/linalg.f90:49-76). Lane (rep, lane) holds entry (lane, rep) of the
+// matrix in `a`; column k is in group k, so group k searches the pivot.
+// Returns 0 on a zero pivot. This is the exact rule in every case (hypot
+// values compared when the |a|^2 screen is inconclusive); lu_one_slot runs
+// it only when its own screen fails.
+template <int G>
+PHC_D int lu_one_slot_exact(int N, cx& a, const bool in, const int col,
+                            int rep, int lane, int& pos, uint64_t& plb) {
+  constexpr int LP = GridShape<G>::LP;
+  static_assert(GridShape<G>::E == 1, "one column per lane");
+  int ok = 1;
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    if (k >= N) break;
+    // column k of this lane's row, fetched while the pivot is searched:
+    // every group forms its row's multiplier itself
+    const cx aik = pshfl_c<LP>(a, k * G + lane);
+    double bv;
+    int bp, bl;
+    pivot_search<G, LP>(in && rep == k && pos >= k, a, pos, lane, k * G, bv,
+                        bp, bl);
+    if (bv == 0.0) {
+      ok = 0;
+      break;
+    }
+    plb |= (uint64_t)bl << (4 * k);
+    const cx piv = pbcast_c<LP>(a, k * G + bl);
+    const cx pr = pshfl_c<LP>(a, rep * G + bl);
+    int np = pos;
+    if (pos == k) np = bp;
+    if (lane == bl) np = k;
+    pos = np;
+    const cx li = cdiv(aik, piv);
+    if (in && rep == k && pos > k) a = li;
+    const cx upd = csub(a, cmul(li, pr));
+    if (in && col > k && pos > k) a = upd;
+  }
+  return ok;
+}
+
+// The same LU, fast. Each pivot search is the |a|^2 screen alone: the
+// largest |a|^2 of the column's candidates (a value-only DPP max), the
+// winner found from a ballot of the candidates that reach it. The screen
+// decides a step when exactly one candidate lies within 1e-12 of that
+// maximum, no candidate is NaN and the maximum is in [1e-280, 1e280]: its
+// hypot is then strictly the largest too, so the step is the exact rule's.
+// The steps run without branches and only record whether every screen was
+// conclusive; if one was not (near ties, NaN, zero pivots, extreme
+// ranges), the whole factorization is redone by lu_one_slot_exact from
+// the saved entries. The entries travel through the path's G x G tile
+// `T` (LDS, free during the LU): T[rep G + lane] = entry (lane, rep).
+template <int G>
+PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
+                      int lane, int& pos, uint64_t& plb, cx* __restrict__ T) {
+  constexpr int LP = GridShape<G>::LP;
+  static_assert(GridShape<G>::E == 1, "one column per lane");
+  // the path's lanes in a ballot (several paths share a wavefront if G = 4)
+  const int shift = LP == 64 ? 0 : (int)(__lane_id() / LP) * LP;
+  const unsigned long long pmask = LP == 64 ? ~0ull : ((1ull << LP) - 1ull);
+  const cx a_in = a;
+  const int pos_in = pos;
+  bool sure = true;
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    if (k >= N) break;
+    wave_lds_fence();
+    T[rep * G + lane] = a;
+    wave_lds_fence();
+    const cx aik = T[k * G + lane];
+    const int gbase = k * G;
+    const bool cand = in && rep == k && pos >= k;
+    const double sq = a.re * a.re + a.im * a.im;
+    const double sv = pbcast_d<LP>(gmax<G>(cand ? sq : -1.0), gbase);
+    const unsigned long long hit = (__ballot(cand && sq == sv) >> shift) & pmask;
+    const unsigned long long near =
+        (__ballot(cand && (sq >= sv * (1.0 - 1e-12) || sq != sq)) >> shift) &
+        pmask;
+    sure = sure && sv > 1e-280 && sv < 1e280 && __popcll(near) == 1;
+    // the winner's lane in the path (any lane of the group if none: the
+    // step is then redone exactly)
+    const int wl = hit ? __ffsll(hit) - 1 : gbase;
+    const int bl = wl - gbase;
+    const int bp = pbcast_i<LP>(pos, wl);
+    plb |= (uint64_t)(bl & 15) << (4 * k);
+    const cx piv = T[k * G + bl];
+    const cx pr = T[rep * G + bl];
+    int np = pos;
+    if (pos == k) np = bp;
+    if (lane == bl) np = k;
+    pos = np;
+    const cx li = cdiv(aik, piv);
+    if (in && rep == k && pos > k) a = li;
+    const cx upd = csub(a, cmul(li, pr));
+    if (in && col > k && pos > k) a = upd;
+  }
+  if (sure) return 1;
+  a = a_in;
+  pos = pos_in;
+  plb = 0;
+  return lu_one_slot_exact<G>(N, a, in, col, rep, lane, pos, plb);
+}
+
 // Forward and back substitution (src/linalg.f90:93-111) for NC rows known
 // at compile time, row per lane: lane i of every group holds row i of the
 // factors and runs the recurrence of that row; the value of the row at
@@ -785,38 +888,7 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   // pivot lanes of the steps: PL[k] (< 16) in bits 4k..4k+3
   uint64_t plb = 0;
   if constexpr (E == 1) {
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-      if (k >= N) break;
-      const int kr = k % RG, ke = k / RG;
-      // column k of this lane's row, fetched while the pivot is searched:
-      // every group forms its row's multiplier itself
-      const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
-      double bv;
-      int bp, bl;
-      pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, lane,
-                          kr * G, bv, bp, bl);
-      if (bv == 0.0) {
-        ok = 0;
-        break;
-      }
-      plb |= (uint64_t)bl << (4 * k);
-      const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
-      cx pr[E];
-#pragma unroll
-      for (int e = 0; e < E; ++e) pr[e] = pshfl_c<LP>(a[e], rep * G + bl);
-      int np = pos;
-      if (pos == k) np = bp;
-      if (lane == bl) np = k;
-      pos = np;
-      const cx li = cdiv(aik, piv);
-      if (in[ke] && rep == kr && pos > k) a[ke] = li;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const cx upd = csub(a[e], cmul(li, pr[e]));
-        if (in[e] && col[e] > k && pos > k) a[e] = upd;
-      }
-    }
+    ok = lu_one_slot<G>(N, a[0], in[0], col[0], rep, lane, pos, plb, W.A);
   } else {
     // the slot count eu = ceil(N / RG) takes two values per class
     const int eu = slots_used<G>(N);
