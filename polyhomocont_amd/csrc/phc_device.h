@@ -1217,25 +1217,59 @@ __global__ void __launch_bounds__(WG)
       if (L.max_units && units >= L.max_units) break;
       const Req& R = STAGED ? *sreq : L.reqs[S.req];
       Machine<G> M{R, S, SV, lane, writer};
-      cx t = mk(0.0, 0.0), dt = mk(0.0, 0.0);
-      if (S.pc != PC_BE) {
-        const double sv = (S.pc == PC_TAN) ? S.se : S.sn;
-        t = seg_t(S, sv);
-        if (S.pc == PC_TAN)
-          dt = (S.sk == 0) ? mk(S.sb - S.sa, 0.0)
-                           : mk(0.0 * t.re - S.sdth * t.im,
-                                0.0 * t.im + S.sdth * t.re);
-      }
+      // the unit's inputs, from the path's state in LDS
+      auto inputs = [&](cx& t, cx& dt, cx& zl) {
+        t = mk(0.0, 0.0);
+        dt = mk(0.0, 0.0);
+        if (S.pc != PC_BE) {
+          const double sv = (S.pc == PC_TAN) ? S.se : S.sn;
+          t = seg_t(S, sv);
+          if (S.pc == PC_TAN)
+            dt = (S.sk == 0) ? mk(S.sb - S.sa, 0.0)
+                             : mk(0.0 * t.re - S.sdth * t.im,
+                                  0.0 * t.im + S.sdth * t.re);
+        }
+        zl = (lane < R.N) ? SV[V_ZQ * G + lane] : mk(0.0, 0.0);
+      };
       cx x;
       int ok;
       double be = 0.0;
-      const cx zl = (lane < R.N) ? SV[V_ZQ * G + lane] : mk(0.0, 0.0);
-      if (GRID)
-        unit_wave<G>(R, STAGED ? swterms : R.wterms, STAGED ? sjobs : R.jobs,
-                     lane, rep, S.pc, zl, t, dt, wl, x, ok, be);
-      else
+      // The fast LU (lu_one_slot, lu_slots_fast) where it leaves the
+      // kernel's registers at two waves per SIMD: all grid kernels but the
+      // 16-lane one reading its program from global memory, which takes
+      // the exact LU directly.
+      constexpr bool FAST_LU = !(G == 16 && !STAGED);
+      if (GRID && !FAST_LU) {
+        cx t, dt, zl;
+        inputs(t, dt, zl);
+        unit_wave<G, true>(R, STAGED ? swterms : R.wterms,
+                           STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
+                           dt, wl, x, ok, be);
+      } else if (GRID) {
+        {
+          cx t, dt, zl;
+          inputs(t, dt, zl);
+          unit_wave<G, false>(R, STAGED ? swterms : R.wterms,
+                              STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
+                              dt, wl, x, ok, be);
+        }
+        if (ok == 2) {
+          // the fast LU's screen was inconclusive: the same unit with the
+          // exact pivot rule in every step (inputs read again from LDS, so
+          // none of them stays in registers through the first unit)
+          __asm__ __volatile__("" ::: "memory");
+          cx t, dt, zl;
+          inputs(t, dt, zl);
+          unit_wave<G, true>(R, STAGED ? swterms : R.wterms,
+                             STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
+                             dt, wl, x, ok, be);
+        }
+      } else {
+        cx t, dt, zl;
+        inputs(t, dt, zl);
         unit<G, SPLIT>(R, STAGED ? sterms : R.terms, lane, rep, S.pc, zl, t,
                        dt, gl, sl, x, ok, be);
+      }
       units++;
       // this pass of the wavefront ran a unit: its lowest active lane counts
       if ((int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)

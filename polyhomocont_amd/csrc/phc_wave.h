@@ -582,10 +582,10 @@ PHC_D int lu_one_slot_exact(int N, cx& a, const bool in, const int col,
 // maximum, no candidate is NaN and the maximum is in [1e-280, 1e280]: its
 // hypot is then strictly the largest too, so the step is the exact rule's.
 // The steps run without branches and only record whether every screen was
-// conclusive; if one was not (near ties, NaN, zero pivots, extreme
-// ranges), the whole factorization is redone by lu_one_slot_exact from
-// the saved entries. The entries travel through the path's G x G tile
-// `T` (LDS, free during the LU): T[rep G + lane] = entry (lane, rep).
+// conclusive: returns 1, or 2 when one was not (near ties, NaN, zero
+// pivots, extreme ranges) -- the caller then redoes the unit with
+// lu_one_slot_exact. The entries travel through the path's G x G tile `T`
+// (LDS, free during the LU): T[rep G + lane] = entry (lane, rep).
 template <int G>
 PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
                       int lane, int& pos, uint64_t& plb, cx* __restrict__ T) {
@@ -594,8 +594,6 @@ PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
   // the path's lanes in a ballot (several paths share a wavefront if G = 4)
   const int shift = LP == 64 ? 0 : (int)(__lane_id() / LP) * LP;
   const unsigned long long pmask = LP == 64 ? ~0ull : ((1ull << LP) - 1ull);
-  const cx a_in = a;
-  const int pos_in = pos;
   bool sure = true;
 #pragma unroll
   for (int k = 0; k < G; ++k) {
@@ -630,11 +628,63 @@ PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
     const cx upd = csub(a, cmul(li, pr));
     if (in && col > k && pos > k) a = upd;
   }
-  if (sure) return 1;
-  a = a_in;
-  pos = pos_in;
-  plb = 0;
-  return lu_one_slot_exact<G>(N, a, in, col, rep, lane, pos, plb);
+  return sure ? 1 : 2;
+}
+
+// lu_slots with the screen of lu_one_slot: the steps run branch-free on
+// the |a|^2 screen, the entries through the path's G x G tile `T`
+// (T[col G + row], LDS, free during the LU). Returns 1, or 2 when a
+// step's screen was inconclusive (the caller then redoes the unit with
+// lu_slots).
+template <int G, int EU>
+PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
+                        const bool (&in)[GridShape<G>::E],
+                        const int (&col)[GridShape<G>::E], int rep, int lane,
+                        int& pos, uint64_t& plb, cx* __restrict__ T) {
+  constexpr int LP = GridShape<G>::LP, RG = GridShape<G>::R,
+                E = GridShape<G>::E;
+  static_assert(LP == 64, "one path per wavefront");
+  bool sure = true;
+  static_for<E>([&](auto ec) {
+    constexpr int ke = decltype(ec)::value;
+    if constexpr (ke < EU) {
+#pragma unroll
+      for (int kr = 0; kr < RG; ++kr) {
+        const int k = ke * RG + kr;
+        if (k >= N) break;
+        wave_lds_fence();
+#pragma unroll
+        for (int e = ke; e < EU; ++e) T[col[e] * G + lane] = a[e];
+        wave_lds_fence();
+        const cx aik = T[k * G + lane];
+        const int gbase = kr * G;
+        const bool cand = in[ke] && rep == kr && pos >= k;
+        const double sq = a[ke].re * a[ke].re + a[ke].im * a[ke].im;
+        const double sv = pbcast_d<LP>(gmax<G>(cand ? sq : -1.0), gbase);
+        const unsigned long long hit = __ballot(cand && sq == sv);
+        const unsigned long long near =
+            __ballot(cand && (sq >= sv * (1.0 - 1e-12) || sq != sq));
+        sure = sure && sv > 1e-280 && sv < 1e280 && __popcll(near) == 1;
+        const int wl = hit ? __ffsll(hit) - 1 : gbase;
+        const int bl = wl - gbase;
+        const int bp = pbcast_i<LP>(pos, wl);
+        plb |= (uint64_t)(bl & 15) << (4 * k);
+        const cx piv = T[k * G + bl];
+        int np = pos;
+        if (pos == k) np = bp;
+        if (lane == bl) np = k;
+        pos = np;
+        const cx li = cdiv(aik, piv);
+        if (in[ke] && rep == kr && pos > k) a[ke] = li;
+#pragma unroll
+        for (int e = ke; e < EU; ++e) {
+          const cx upd = csub(a[e], cmul(li, T[col[e] * G + bl]));
+          if (in[e] && col[e] > k && pos > k) a[e] = upd;
+        }
+      }
+    }
+  });
+  return sure ? 1 : 2;
 }
 
 // Forward and back substitution (src/linalg.f90:93-111) for NC rows known
@@ -705,7 +755,7 @@ PHC_D cx solve_dispatch(int N, const cx (&arow)[G], const cx bown,
 // wavefront, and skips it elsewhere. Either way the result is the same up
 // to the sign of a zero component. Zero padding of sums is exact: the
 // sums start at +0 and so are never -0.
-template <int G>
+template <int G, bool EXACT>
 PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
                      const uint4* __restrict__ jobs, const int lane,
                      const int rep, const int mode, const cx zq, const cx t,
@@ -888,17 +938,28 @@ PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
   // pivot lanes of the steps: PL[k] (< 16) in bits 4k..4k+3
   uint64_t plb = 0;
   if constexpr (E == 1) {
-    ok = lu_one_slot<G>(N, a[0], in[0], col[0], rep, lane, pos, plb, W.A);
+    if constexpr (EXACT)
+      ok = lu_one_slot_exact<G>(N, a[0], in[0], col[0], rep, lane, pos, plb);
+    else
+      ok = lu_one_slot<G>(N, a[0], in[0], col[0], rep, lane, pos, plb, W.A);
   } else {
     // the slot count eu = ceil(N / RG) takes two values per class
     const int eu = slots_used<G>(N);
-    if (eu == E)
-      ok = lu_slots<G, E>(N, a, in, col, rep, lane, pos, plb);
-    else
-      ok = lu_slots<G, (E > 1 ? E - 1 : 1)>(N, a, in, col, rep, lane, pos, plb);
+    constexpr int E1 = E > 1 ? E - 1 : 1;
+    if constexpr (EXACT) {
+      if (eu == E)
+        ok = lu_slots<G, E>(N, a, in, col, rep, lane, pos, plb);
+      else
+        ok = lu_slots<G, E1>(N, a, in, col, rep, lane, pos, plb);
+    } else {
+      if (eu == E)
+        ok = lu_slots_fast<G, E>(N, a, in, col, rep, lane, pos, plb, W.A);
+      else
+        ok = lu_slots_fast<G, E1>(N, a, in, col, rep, lane, pos, plb, W.A);
+    }
   }
   okout = ok;
-  if (!ok) {
+  if (ok != 1) {  // 0: zero pivot; 2: the fast LU's screen was inconclusive
     xout = mk(0.0, 0.0);
     return;
   }
