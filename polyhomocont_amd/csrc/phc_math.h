@@ -15,6 +15,17 @@
 //  * complex ** integer  -> flang runtime's binary powering (squarings of
 //    the base, lowest exponent bit first).
 //
+// Provenance and licences of the methods restated here (no upstream code is
+// included; the algorithms and published constants are restated so that
+// results match those libraries bit for bit):
+//  * complex division: the algorithm of LLVM compiler-rt's __divdc3
+//    (compiler-rt is Apache-2.0 WITH LLVM-exception);
+//  * hypot: C. F. Borges, arXiv:1904.09481, as used by glibc 2.35
+//    (glibc is LGPL-2.1-or-later);
+//  * sin/cos: the IBM Accurate Mathematical Library method of glibc 2.35's
+//    sysdeps/ieee754/dbl-64 (LGPL-2.1-or-later), coefficients as published
+//    there; the 1/128 table (phc_sincos_table.h) is generated from its
+//    definition by gen_sincos_table.py.
 // Each of these was checked bit for bit against the host routines on
 // millions of random inputs (tests/test_math_host.py runs the same checks
 // through this header compiled for the CPU). Everything is __host__
