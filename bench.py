@@ -51,6 +51,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector, dense
 UNITS_PER_LAUNCH = 64
+SUBMITS_PER_CALL = 32
 
 
 def log(msg):
@@ -85,8 +86,9 @@ def prepare(H, usys, opts):
     """What `solve` does before tracking (src/solve.f90:166-190)."""
     from polyhomocont_amd import abi
     g, b, a, _ = H.homotopy(usys, opts.seed)
-    return (H.pack(usys), abi.HomotopyData(g, b, a), opts,
-            H.start_points(usys, opts.seed))
+    start = H.start_points(usys, opts.seed)
+    return (H.pack(usys), abi.HomotopyData(g, b, a), opts, start,
+            abi.EndpointArrays(start.shape[0], usys.n + 1))
 
 
 class Engine:
@@ -122,14 +124,20 @@ class Engine:
             while nxt < R and arr[nxt] <= now:
                 prep[nxt] = self.pool.submit(prepare, self.H, stream[nxt], opts)
                 nxt += 1
-            for i in [i for i, f in prep.items() if f.done()]:
+            # a bounded number of submissions between progress calls, so
+            # that rounds that end meanwhile are restarted promptly
+            ready = [i for i, f in prep.items() if f.done()][:SUBMITS_PER_CALL]
+            check = False
+            for i in ready:
                 req = prep.pop(i).result()
                 t, out = self.srv.submit(*req)
                 cls[i] = self.srv.ticket_class(t)
                 inflight[t] = (i, out, req[0])
+                check = check or out.npaths == 0
             if inflight:
-                self.srv.progress()
-                for t in [t for t in inflight if self.srv.done(t)]:
+                # tickets are polled only after a call that finished some
+                check = self.srv.progress() > 0 or check
+                for t in [t for t in inflight if check and self.srv.done(t)]:
                     i, out, packed = inflight.pop(t)
                     self.srv.release(t)
                     recs[i] = (packed, out)

@@ -334,11 +334,16 @@ int phc_server_submit(phc_server *srv, const phc_system *sys,
                       const phc_homotopy *hom, const phc_solve_options *opts,
                       int64_t npaths, const double *start, phc_endpoints *out,
                       uint64_t *ticket);
-/* One step of the engine: waits for the round of launches in flight (if
- * any), retires the requests it finished (records copied to their `out`),
- * admits queued requests that fit, launches the next round and returns
- * while it runs, so the caller's host work until the next call overlaps
- * the GPU. Returns the number of requests completed by this call. */
+/* One step of the engine. Every lanes class runs its own rounds of
+ * launches; a call retires each class whose round has ended (records of
+ * the requests it finished copied to their `out`), admits queued requests
+ * that fit into the classes between rounds (FIFO within a class; a request
+ * that waits holds back only later requests of its class), starts the
+ * next round of every idle class with work and returns while the rounds
+ * run, so the caller's host work until the next call overlaps the GPU.
+ * When no round has ended and none could start, it first waits for one to
+ * end (without holding the context). Returns the number of requests
+ * completed by this call. */
 int phc_server_progress(phc_server *srv);
 /* PHC_OK when the request's records are in its `out`, PHC_ERR_PENDING if
  * not yet; phc_server_wait blocks (driving progress) until done. */
@@ -349,7 +354,9 @@ int phc_server_wait(phc_server *srv, uint64_t ticket);
  * are on the device; PHC_ERR_INVALID_INPUT for an unknown ticket. A
  * long-running server calls it once per request, after poll/wait. */
 int phc_server_release(phc_server *srv, uint64_t ticket);
-/* Per lanes class of the engine (returns their number, at most 8): out[8c
+#define PHC_SERVER_MAX_CLASSES 16
+/* Per lanes class of the engine (returns their number, at most
+ * PHC_SERVER_MAX_CLASSES; out holds 8 per class): out[8c
  * + 0] lanes per group G, [1] mapping (0 row, 2 grid), [2] kernel launches
  * so far, [3] their kernel time in ms (HIP events around each round on the
  * class's stream), [4] linearization units run, [5] unit slots offered

@@ -12,6 +12,7 @@ from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libphc_hip.so")
+SERVER_MAX_CLASSES = 16  # PHC_SERVER_MAX_CLASSES of include/phc.h
 
 _lib = None
 
@@ -239,11 +240,15 @@ class Server:
         self.h = h
         self._keep = {}
 
-    def submit(self, packed, hom, opts, start):
+    def submit(self, packed, hom, opts, start, out=None):
         """Queues a request; returns (ticket, EndpointArrays filled on
-        completion)."""
+        completion). `out`, when given, is the caller's EndpointArrays for
+        the request's paths (made beforehand, e.g. on a worker thread)."""
         start = np.ascontiguousarray(start, np.complex128)
-        out = abi.EndpointArrays(start.shape[0], packed.n + 1)
+        if out is None:
+            out = abi.EndpointArrays(start.shape[0], packed.n + 1)
+        elif out.npaths != start.shape[0] or out.N != packed.n + 1:
+            raise ValueError("out does not match the request's paths")
         t = C.c_uint64()
         self.ctx._check(self.ctx.L.phc_server_submit(
             self.h, C.byref(packed.c), C.byref(hom.c), C.byref(opts),
@@ -253,6 +258,9 @@ class Server:
         return t.value, out
 
     def progress(self):
+        """Retires the classes whose round ended, admits and starts the
+        next rounds; waits for a round to end only when none has and none
+        could start. Returns the number of requests finished."""
         rc = self.ctx.L.phc_server_progress(self.h)
         if rc < 0:
             self.ctx._check(rc, "phc_server_progress")
@@ -277,7 +285,7 @@ class Server:
     def stats(self):
         """Per lanes class: G, mapping, launches, kernel ms, units, unit
         slots, resident path slots."""
-        out = np.zeros(64)
+        out = np.zeros(8 * SERVER_MAX_CLASSES)
         n = self.ctx.L.phc_server_stats(self.h, out.ctypes.data)
         if n < 0:
             self.ctx._check(n, "phc_server_stats")
