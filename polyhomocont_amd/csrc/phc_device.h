@@ -137,12 +137,10 @@ PHC_HD constexpr size_t state_lds_bytes(int G) {
          (size_t)NVEC * G * sizeof(cx);
 }
 
-// Saved state of one lane of a path between launches of the resumable
-// kernel (serving engine): the scalars and the lane's components.
-struct PathState {
-  PathScalars s;
-  cx v[NVEC];
-};
+// Between launches of the resumable kernel (serving engine) a path slot
+// keeps its LDS state block, state_lds_bytes(G) bytes, in HBM as it is in
+// LDS: the path's lanes copy it 16 bytes each. An idle slot saves only
+// its program counter (its next path starts from scratch).
 
 // A queued path of the serving engine: request slot and path index.
 struct QItem {
@@ -163,7 +161,8 @@ struct Launch {
   const QItem* items;
   unsigned long long cap;
   const unsigned long long* tail;
-  PathState* slots;         // [groups][G] saved state (resumable launches)
+  char* slots;              // [slots][state_lds_bytes(G)] saved states
+                            // (resumable launches)
   int32_t resume;           // load saved states at entry
   int32_t max_units;        // units per group per launch, 0 = unlimited
   unsigned int* active;     // groups still holding a path at exit
@@ -238,6 +237,24 @@ PHC_D void gargmax(double& v, int& p, int& l) {
 template <int G>
 PHC_D double gnorm(cx v, int lane, int N) {
   return gmax<G>(lane < N ? cabs_(v) : 0.0);
+}
+
+// A pointer read from memory (a request's tables) is generic to the
+// compiler, so its loads become flat loads, which count on the LDS wait
+// counter too: every LDS wait in the unit would then also wait for them,
+// the job records prefetched for the next step included. Passing the
+// tables as global-typed pointers (the unit's table arguments are template
+// parameters) makes them global loads.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+// A unit's table: the copy staged in LDS, or the request's own in global
+// memory.
+template <bool STAGED, class T>
+PHC_D auto table(T* staged, const T* global) {
+  if constexpr (STAGED)
+    return (const T*)staged;
+  else
+    return (gptr<T>)global;
 }
 
 PHC_D void wave_lds_fence() {
@@ -1148,12 +1165,12 @@ __global__ void __launch_bounds__(WG)
     if (SPLIT) sl.bind(mine + GroupLds<G>::bytes(L.expmax));
   }
 
-  // every lane of the path writes the same scalars and its own components
+  // the path's state block, saved by the previous launch
+  constexpr int SQ = (int)(state_lds_bytes(G) / 16);
   if (L.resume) {
-    const PathState& g = L.slots[(size_t)slot * G + lane];
-    S = g.s;
-#pragma unroll
-    for (int k = 0; k < NVEC; ++k) SV[k * G + lane] = g.v[k];
+    const uint4* src = (const uint4*)(L.slots + (size_t)slot * state_lds_bytes(G));
+    for (int i = (int)(threadIdx.x % LP); i < SQ; i += LP)
+      ((uint4*)pbase)[i] = src[i];
   } else {
     S.pc = PC_IDLE;
   }
@@ -1242,15 +1259,15 @@ __global__ void __launch_bounds__(WG)
       if (GRID && !FAST_LU) {
         cx t, dt, zl;
         inputs(t, dt, zl);
-        unit_wave<G, true>(R, STAGED ? swterms : R.wterms,
-                           STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
+        unit_wave<G, true>(R, table<STAGED>(swterms, R.wterms),
+                           table<STAGED>(sjobs, R.jobs), lane, rep, S.pc, zl, t,
                            dt, wl, x, ok, be);
       } else if (GRID) {
         {
           cx t, dt, zl;
           inputs(t, dt, zl);
-          unit_wave<G, false>(R, STAGED ? swterms : R.wterms,
-                              STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
+          unit_wave<G, false>(R, table<STAGED>(swterms, R.wterms),
+                              table<STAGED>(sjobs, R.jobs), lane, rep, S.pc, zl, t,
                               dt, wl, x, ok, be);
         }
         if (ok == 2) {
@@ -1260,8 +1277,8 @@ __global__ void __launch_bounds__(WG)
           __asm__ __volatile__("" ::: "memory");
           cx t, dt, zl;
           inputs(t, dt, zl);
-          unit_wave<G, true>(R, STAGED ? swterms : R.wterms,
-                             STAGED ? sjobs : R.jobs, lane, rep, S.pc, zl, t,
+          unit_wave<G, true>(R, table<STAGED>(swterms, R.wterms),
+                             table<STAGED>(sjobs, R.jobs), lane, rep, S.pc, zl, t,
                              dt, wl, x, ok, be);
         }
       } else {
@@ -1286,10 +1303,14 @@ __global__ void __launch_bounds__(WG)
     M.run();
   }
   if (L.slots) {
-    PathState& g = L.slots[(size_t)slot * G + lane];
-    g.s = S;
-#pragma unroll
-    for (int k = 0; k < NVEC; ++k) g.v[k] = SV[k * G + lane];
+    wave_lds_fence();
+    char* dst = L.slots + (size_t)slot * state_lds_bytes(G);
+    if (S.pc == PC_IDLE) {
+      if (threadIdx.x % LP == 0) ((PathScalars*)dst)->pc = PC_IDLE;
+    } else {
+      for (int i = (int)(threadIdx.x % LP); i < SQ; i += LP)
+        ((uint4*)dst)[i] = ((const uint4*)pbase)[i];
+    }
   }
   if (writer && S.pc != PC_IDLE && lane == 0 && L.active) atomicAdd(L.active, 1u);
   if (writer && lane == 0 && L.counters) {

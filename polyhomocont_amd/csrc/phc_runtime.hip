@@ -469,7 +469,6 @@ struct ClassRun {
   unsigned long long* d_next = nullptr;
   unsigned long long* d_counters = nullptr;
   unsigned int* d_active = nullptr;
-  PathState* d_slots = nullptr;
   int64_t total = 0;
   int expmax = 1;
   int blocks = 0, threads = 0;
@@ -643,8 +642,6 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
     need += Arena::al(cr.reqs.size() * sizeof(Req));
     need += Arena::al((cr.reqs.size() + 1) * sizeof(int64_t));
     need += 256 * 3;
-    need += Arena::al((size_t)cr.blocks * (cr.threads / cr.lp) * cr.G *
-                      sizeof(PathState));
     plans.push_back(cr);
   }
   HIPCHK(ctx, hipMalloc(&st->arena.base, need));
@@ -709,9 +706,6 @@ int stage_requests(phc_ctx* ctx, phc_staged* st) {
     cr.d_next = (unsigned long long*)arena_take(st->arena, 8);
     cr.d_counters = (unsigned long long*)arena_take(st->arena, 32);
     cr.d_active = (unsigned int*)arena_take(st->arena, 4);
-    cr.d_slots = (PathState*)arena_take(
-        st->arena,
-        (size_t)cr.blocks * (cr.threads / cr.lp) * cr.G * sizeof(PathState));
     st->classes.push_back(cr);
   }
   return PHC_OK;

@@ -46,6 +46,29 @@ struct WTerm {
 };
 static_assert(sizeof(WTerm) == 32, "staged copies move terms in 16-byte units");
 
+// Table records through either kind of table pointer (staged in LDS, or
+// global-typed): the records' copy constructors take generic references,
+// so a global-typed record is read as plain 16-byte vectors.
+typedef unsigned int phc_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long phc_u64x2 __attribute__((ext_vector_type(2)));
+PHC_D uint4 ld(const uint4* p, size_t i) { return p[i]; }
+PHC_D uint4 ld(gptr<uint4> p, size_t i) {
+  const phc_u32x4 v = ((gptr<phc_u32x4>)p)[i];
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+PHC_D WTerm ld(const WTerm* p, size_t i) { return p[i]; }
+PHC_D WTerm ld(gptr<WTerm> p, size_t i) {
+  const gptr<phc_u64x2> q = (gptr<phc_u64x2>)p + 2 * i;
+  const phc_u64x2 a = q[0], b = q[1];
+  WTerm w;
+  w.re = __longlong_as_double((long long)a.x);
+  w.im = __longlong_as_double((long long)a.y);
+  w.off = b.x;
+  w.nnz = (uint32_t)b.y;
+  w.slot = (uint32_t)(b.y >> 32);
+  return w;
+}
+
 // Job records of lane (i, j): the terms of equation i with a nonzero
 // exponent e of z_j, in term order. Two layouts (uint4):
 //  short (every term has at most 4 nonzero exponents):
@@ -176,30 +199,30 @@ PHC_D cx term_value(const WTerm& w, const cx* __restrict__ T) {
   for (int a = 1; a < FV; ++a) P = cmul(P, f[a]);
   return cmul(mk(w.re, w.im), P);
 }
-template <int LP, int FV>
-PHC_D void term_values(const Req& R, const WTerm* __restrict__ wt,
+template <int LP, int FV, class WP>
+PHC_D void term_values(const Req& R, WP wt,
                        const cx* __restrict__ T, cx* __restrict__ V, int q) {
   const int nt = R.nterms;
   int k = q;
   for (; k + LP < nt; k += 2 * LP) {
-    const WTerm w0 = wt[k], w1 = wt[k + LP];
+    const WTerm w0 = ld(wt, k), w1 = ld(wt, k + LP);
     const cx v0 = term_value<FV>(w0, T);
     const cx v1 = term_value<FV>(w1, T);
     V[w0.slot] = v0;
     V[w1.slot] = v1;
   }
   if (k < nt) {
-    const WTerm w0 = wt[k];
+    const WTerm w0 = ld(wt, k);
     V[w0.slot] = term_value<FV>(w0, T);
   }
 }
 
 // One derivative c e pw(e-1, j) pre(j-1) suf(j+1) (src/sparse.f90:273) of
 // a short-layout job record, FL factors per prefix and suffix.
-template <int FL>
-PHC_D cx job_product(const uint4 job, const WTerm* __restrict__ wt,
+template <int FL, class WP>
+PHC_D cx job_product(const uint4 job, WP wt,
                      const cx* __restrict__ T) {
-  const WTerm& w = wt[job.x & 0xFFFFFu];
+  const auto& w = wt[job.x & 0xFFFFFu];
   const cx c = mk(w.re, w.im);
   const cx B = T[byte_of(job.y, 0)];
   cx Pf[FL], Sf[FL];
@@ -224,9 +247,9 @@ PHC_D cx job_product(const uint4 job, const WTerm* __restrict__ wt,
 // are padded to an even count), the next turn's records loaded while this
 // one computes. Job table [m][LP]. Grids with several slots per lane use
 // the run layout (jac_runs) instead.
-template <int G, int FL>
-PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
-                     const uint4* __restrict__ jobs, const cx* __restrict__ T,
+template <int G, int FL, class WP, class JP>
+PHC_D void jac_short(const Req& R, WP wt,
+                     JP jobs, const cx* __restrict__ T,
                      const cx* __restrict__ V, int q, int vbase, cx& acc) {
   static_assert(GridShape<G>::E == 1, "one slot per lane");
   constexpr int LP = GridShape<G>::LP, U = 2;
@@ -235,14 +258,14 @@ PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
   uint4 jn[U];
 #pragma unroll
   for (int u = 0; u < U; ++u)
-    if (u < M) jn[u] = jobs[(size_t)u * LP + q];
+    if (u < M) jn[u] = ld(jobs, (size_t)u * LP + q);
   for (int m = 0; m < M; m += U) {
     uint4 jc[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) jc[u] = jn[u];
     if (m + U < M) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) jn[u] = jobs[(size_t)(m + U + u) * LP + q];
+      for (int u = 0; u < U; ++u) jn[u] = ld(jobs, (size_t)(m + U + u) * LP + q);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -266,9 +289,9 @@ PHC_D void jac_short(const Req& R, const WTerm* __restrict__ wt,
 // jac_short does -- and at a run's last step (record w bit 31) stores the
 // sum to S[w bits 16..30] and starts the next run from +0. Table
 // [m][LP], two steps per turn with the next turn's records prefetched.
-template <int FL>
-PHC_D void jac_runs(const Req& R, const WTerm* __restrict__ wt,
-                    const uint4* __restrict__ jobs, const cx* __restrict__ T,
+template <int FL, class WP, class JP>
+PHC_D void jac_runs(const Req& R, WP wt,
+                    JP jobs, const cx* __restrict__ T,
                     const cx* __restrict__ V, int q, cx* __restrict__ S) {
   constexpr int LP = 64, U = 2;
   const int M = R.njobs;
@@ -276,14 +299,14 @@ PHC_D void jac_runs(const Req& R, const WTerm* __restrict__ wt,
   uint4 jn[U];
 #pragma unroll
   for (int u = 0; u < U; ++u)
-    if (u < M) jn[u] = jobs[(size_t)u * LP + q];
+    if (u < M) jn[u] = ld(jobs, (size_t)u * LP + q);
   for (int m = 0; m < M; m += U) {
     uint4 jc[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) jc[u] = jn[u];
     if (m + U < M) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) jn[u] = jobs[(size_t)(m + U + u) * LP + q];
+      for (int u = 0; u < U; ++u) jn[u] = ld(jobs, (size_t)(m + U + u) * LP + q);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -308,13 +331,13 @@ PHC_D void jac_runs(const Req& R, const WTerm* __restrict__ wt,
 // products padded with pw(0, 0) = 1 to FLG factors each (the short layout
 // pads the same way); FLG = max(maxp, maxs) is a template argument, so the
 // factor loops carry no runtime bound.
-template <int FLG>
-PHC_D cx job_product_general(const uint4 job, const WTerm* __restrict__ wt,
+template <int FLG, class WP>
+PHC_D cx job_product_general(const uint4 job, WP wt,
                              const cx* __restrict__ T) {
   const int ex = (int)((job.x >> 20) & 255u);
   const uint64_t off = ((uint64_t)job.z << 32) | job.y;
   const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
-  const WTerm& w = wt[job.x & 0xFFFFFu];
+  const auto& w = wt[job.x & 0xFFFFFu];
   const cx c = mk(w.re, w.im);
   const cx B = T[woff(off, 0)];
   cx Pf[FLG], Sf[FLG];
@@ -337,9 +360,9 @@ PHC_D cx job_product_general(const uint4 job, const WTerm* __restrict__ wt,
 // jobs per turn, next turn's records prefetched, four term values per
 // step) around job_product_general. Rows past a lane's last job are the
 // zero-coefficient term with no other factors, so they add +0.
-template <int G, int FLG>
-PHC_D void jac_general1(const Req& R, const WTerm* __restrict__ wt,
-                        const uint4* __restrict__ jobs,
+template <int G, int FLG, class WP, class JP>
+PHC_D void jac_general1(const Req& R, WP wt,
+                        JP jobs,
                         const cx* __restrict__ T, const cx* __restrict__ V,
                         int q, int vbase, cx& acc) {
   constexpr int LP = GridShape<G>::LP, U = 2;
@@ -348,14 +371,14 @@ PHC_D void jac_general1(const Req& R, const WTerm* __restrict__ wt,
   uint4 jn[U];
 #pragma unroll
   for (int u = 0; u < U; ++u)
-    if (u < M) jn[u] = jobs[(size_t)u * LP + q];
+    if (u < M) jn[u] = ld(jobs, (size_t)u * LP + q);
   for (int m = 0; m < M; m += U) {
     uint4 jc[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) jc[u] = jn[u];
     if (m + U < M) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) jn[u] = jobs[(size_t)(m + U + u) * LP + q];
+      for (int u = 0; u < U; ++u) jn[u] = ld(jobs, (size_t)(m + U + u) * LP + q);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -373,10 +396,8 @@ PHC_D void jac_general1(const Req& R, const WTerm* __restrict__ wt,
 
 // jac_general1 for FLG = max(maxp, maxs) among NC .. 7 (a term has at
 // most 8 nonzero exponents, so a job has at most 7 other factors).
-template <int G, int NC>
-PHC_D void jac_general1_dispatch(int flg, const Req& R,
-                                 const WTerm* __restrict__ wt,
-                                 const uint4* __restrict__ jobs,
+template <int G, int NC, class WP, class JP>
+PHC_D void jac_general1_dispatch(int flg, const Req& R, WP wt, JP jobs,
                                  const cx* __restrict__ T,
                                  const cx* __restrict__ V, int q, int vbase,
                                  cx& acc) {
@@ -390,9 +411,9 @@ PHC_D void jac_general1_dispatch(int flg, const Req& R,
 }
 
 // The same with the general job layout (any number of factors up to 8).
-template <int G>
-PHC_D void jac_general(const Req& R, const WTerm* __restrict__ wt,
-                       const uint4* __restrict__ jobs, const cx* __restrict__ T,
+template <int G, class WP, class JP>
+PHC_D void jac_general(const Req& R, WP wt,
+                       JP jobs, const cx* __restrict__ T,
                        const cx* __restrict__ V, int q,
                        const int (&vbase)[GridShape<G>::E],
                        const int (&vcount)[GridShape<G>::E],
@@ -408,12 +429,12 @@ PHC_D void jac_general(const Req& R, const WTerm* __restrict__ wt,
     if (e >= eu) return;
     int fk = 0;
     for (int m = 0; m < M; ++m) {
-      const uint4 job = jobs[((size_t)m * LP + q) * E + e];
+      const uint4 job = ld(jobs, ((size_t)m * LP + q) * E + e);
       if (job.x != WJOB_NONE) {
         const int ex = (int)((job.x >> 20) & 255u);
         const uint64_t off = ((uint64_t)job.z << 32) | job.y;
         const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
-        const WTerm& w = wt[job.x & 0xFFFFFu];
+        const auto& w = wt[job.x & 0xFFFFFu];
         const cx c = mk(w.re, w.im);
         const cx B = T[woff(off, 0)];
         cx Pf[FM], Sf[FM];
@@ -755,9 +776,9 @@ PHC_D cx solve_dispatch(int N, const cx (&arow)[G], const cx bown,
 // wavefront, and skips it elsewhere. Either way the result is the same up
 // to the sign of a zero component. Zero padding of sums is exact: the
 // sums start at +0 and so are never -0.
-template <int G, bool EXACT>
-PHC_D void unit_wave(const Req& R, const WTerm* __restrict__ wt,
-                     const uint4* __restrict__ jobs, const int lane,
+template <int G, bool EXACT, class WP, class JP>
+PHC_D void unit_wave(const Req& R, WP wt,
+                     JP jobs, const int lane,
                      const int rep, const int mode, const cx zq, const cx t,
                      const cx dt, const WaveLds<G>& W, cx& xout, int& okout,
                      double& be) {
