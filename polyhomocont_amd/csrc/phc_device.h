@@ -1183,20 +1183,20 @@ __global__ void __launch_bounds__(WG)
   for (;;) {
     if (S.pc == PC_IDLE) {
       if (L.max_units && units >= L.max_units) break;
-      // pop the next queue index; -1 when the queue is empty
+      // pop the next queue index; -1 when the queue is empty. One
+      // fetch-and-add, never retried (a compare-and-swap loop serializes the
+      // slots that pop together, at every launch's start, for milliseconds).
+      // Pops that race past the end leave the head beyond the tail, by at
+      // most one per slot and launch; those indices hold no item (the
+      // engine appends its next items from the head).
       long long idx = -1;
       if (threadIdx.x % LP == 0) {
         const unsigned long long end =
             L.items ? __atomic_load_n(L.tail, __ATOMIC_RELAXED)
                     : (unsigned long long)L.total;
-        unsigned long long h = __atomic_load_n(L.next, __ATOMIC_RELAXED);
-        while (h < end) {
-          const unsigned long long old = atomicCAS(L.next, h, h + 1ull);
-          if (old == h) {
-            idx = (long long)h;
-            break;
-          }
-          h = old;
+        if (__atomic_load_n(L.next, __ATOMIC_RELAXED) < end) {
+          const unsigned long long h = atomicAdd(L.next, 1ull);
+          if (h < end) idx = (long long)h;
         }
       }
       idx = __shfl(idx, 0, LP);
