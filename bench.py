@@ -276,16 +276,19 @@ def serving_section(ctx, H, args, out):
          "lane_occupancy": round(x["units"] / x["unit_slots"], 4)
          if x["unit_slots"] else None}
         for i, x in enumerate(delta)]
+    serving = {"engine_paths_per_s": round(value, 1)}
+    if args.no_plain:
+        eng.close()
+        return stream, res, serving
     # plain per-solve path, saturation
     plain_res, _, plain_s = serve_plain(H, stream, opts)
     log(f"plain per-solve path: {plain_s:.2f} s")
     identical = sum(same_result(a, b) for a, b in zip(res, plain_res))
-    serving = {
-        "engine_paths_per_s": round(value, 1),
+    serving.update({
         "plain_paths_per_s": round(paths / plain_s, 1),
         "speedup_vs_plain": round(value / (paths / plain_s), 3),
         "results_identical_to_plain": f"{identical}/{len(stream)}",
-    }
+    })
     # latency at 70% of the plain saturation rate, then a sweep
     plain_rate = len(stream) / plain_s
     rng = np.random.default_rng(105)
@@ -474,6 +477,53 @@ def c2_section(ctx, H, args):
     return out
 
 
+# ---------------------------------------------------------------------- C3
+def c3_section(ctx, H, args):
+    """The end game stress system (n = 7, 5040 paths, most at infinity,
+    winding numbers > 1): one plain per-solve launch with inputs resident in
+    HBM, its roofline, and the longest path's share."""
+    import numpy as np
+    from polyhomocont_amd import abi, device, flops, workloads
+    usys = workloads.config_system("c3")
+    opts = abi.default_options()
+    gamma, b, patch, npaths = H.homotopy(usys, opts.seed)
+    start = H.start_points(usys, opts.seed)
+    packed = H.pack(usys)
+    st = device.Staged(ctx, packed, abi.HomotopyData(gamma, b, patch), opts,
+                       start)
+    t0 = time.perf_counter()
+    st.run()
+    kernel_ms = st.elapsed_ms()
+    wall = time.perf_counter() - t0
+    rec = st.fetch()
+    info = st.launch_info()
+    c = ctx.counters()
+    st.close()
+    units = rec.nlin[:npaths] + rec.nbe[:npaths]
+    fl = int(flops.path_flops(packed, rec.nlin[:npaths], rec.nbe[:npaths]).sum())
+    achieved = fl / (kernel_ms / 1e3) / 1e12
+    longest = int(np.argmax(units))
+    return {
+        "workload": "cyclic-like n=7 (5040 paths, most at infinity), plain "
+                    "tracker, inputs resident in HBM",
+        "seconds": round(wall, 3), "paths_per_s": round(npaths / wall, 2),
+        "launch": info,
+        "roofline": {"bound": "fp64-valu", "achieved": round(achieved, 5),
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 7),
+                     "kernel": "phc::track_kernel<%d, %s, %d>" % (
+                         info["group"], "true" if info["staged"] else "false",
+                         {"row": 0, "split": 1, "grid": 2}[info["mapping"]]),
+                     "flops_per_launch": fl,
+                     "avg_launch_ms": round(kernel_ms, 3)},
+        "units": int(units.sum()),
+        "longest_path": {"path": longest, "units": int(units[longest]),
+                         "us_per_unit": round(kernel_ms * 1e3 /
+                                              max(int(units[longest]), 1), 3)},
+        "slot_time_utilisation": round(c["slot_utilisation"], 5),
+    }
+
+
 # ------------------------------------------------------------------- batch
 def batch_section(ctx, H, count):
     from polyhomocont_amd import abi, workloads
@@ -486,9 +536,15 @@ def batch_section(ctx, H, count):
     paths = sum(len(it[3]) for it in items)
     log(f"batch: {count} systems, {paths} paths")
     t0 = time.perf_counter()
+    tuned = ctx.autotune_batch(items)
+    tune_s = time.perf_counter() - t0
+    log(f"batch autotune ({tune_s:.2f} s): {tuned}")
+    t0 = time.perf_counter()
     outs = ctx.track_batch(items)
     dt = time.perf_counter() - t0
     c = ctx.counters()
+    plan = ctx.last_plan()
+    ctx.clear_tuning()
     log(f"batch: {dt:.2f} s")
     return {"workload": f"{count} random sparse systems (n 3..8, degree 2..4, "
                         "5..40 terms per equation), one launch per lanes class",
@@ -496,8 +552,12 @@ def batch_section(ctx, H, count):
             "paths_per_s": round(paths / dt, 1), "seconds": round(dt, 3),
             "units": c["units"],
             "lane_occupancy_after_divergence": round(c["occupancy"], 4),
-            "includes": "host packing excluded; upload, kernels, download "
-                        "included"}
+            "slot_time_utilisation": round(c["slot_utilisation"], 4),
+            "longest_slot_units": c["max_slot_units"],
+            "autotuned": tuned, "autotune_seconds": round(tune_s, 2),
+            "launch_plan": plan,
+            "includes": "host packing and autotuning excluded; upload, "
+                        "kernels, download included"}
 
 
 def main():
@@ -513,6 +573,12 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="systems of the batched multi-system launch (0: skip)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="serving: engine passes only (no plain per-solve "
+                         "path, no latency runs); for profiling the engine")
+    ap.add_argument("--only", choices=("serving", "c2", "batch", "c3"),
+                    help="run one section alone (c3, the 5040-path end game "
+                         "stress solve, runs only when named here)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; the path "
@@ -537,21 +603,27 @@ def main():
         "dtype": "c128 (fp64)",
         "data": "synthetic (seeded complex-Gaussian coefficients)",
     }
-    stream, results, serving = serving_section(ctx, H, args, out)
-    out["serving"] = serving
-    if not args.no_cpu:
-        log("cpu baseline (reference solve on the stream)")
-        cb, par = cpu_stream_baseline(stream, results, args.cpu_seconds)
-        out["cpu_baseline"] = cb
-        out["cpu_baseline_all_cores"] = cpu_stream_all_cores(
-            len(stream), par["requests_compared"], args.cpu_seconds)
-        out["parity"] = par
-        out["gpu_vs_cpu_core"] = round(out["value"] / cb["value"], 1)
-    if not args.no_c2:
+    want = (lambda name: args.only == name) if args.only else (
+        lambda name: name != "c3")
+    if want("serving"):
+        stream, results, serving = serving_section(ctx, H, args, out)
+        out["serving"] = serving
+        if not args.no_cpu:
+            log("cpu baseline (reference solve on the stream)")
+            cb, par = cpu_stream_baseline(stream, results, args.cpu_seconds)
+            out["cpu_baseline"] = cb
+            out["cpu_baseline_all_cores"] = cpu_stream_all_cores(
+                len(stream), par["requests_compared"], args.cpu_seconds)
+            out["parity"] = par
+            out["gpu_vs_cpu_core"] = round(out["value"] / cb["value"], 1)
+    if want("c2") and not args.no_c2:
         log("c2 plain tracker")
         out["c2"] = c2_section(ctx, H, args)
-    if args.batch_systems > 0:
+    if want("batch") and args.batch_systems > 0:
         out["batch"] = batch_section(ctx, H, args.batch_systems)
+    if want("c3"):
+        log("c3 end game stress solve")
+        out["c3"] = c3_section(ctx, H, args)
     ctx.close()
     print(json.dumps(out), flush=True)
 

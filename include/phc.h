@@ -198,8 +198,8 @@ int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
 int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
 
 /* Lane mapping of later tracking calls. 0 chooses per launch: an
- * autotuned choice (phc_ctx_autotune) for single-request launches of a
- * tuned shape, else the min(G*G, 64)-lane grid (a path's linear system one
+ * autotuned choice (phc_ctx_autotune, phc_ctx_autotune_batch) for lanes
+ * classes of a tuned shape, else the min(G*G, 64)-lane grid (a path's linear system one
  * entry per lane, up to four per lane for G = 16, its evaluation spread
  * over all lanes) when the request fits the grid tables, else one path per
  * G-lane group. 1 forces one path per G-lane group (row per lane), 2 one
@@ -303,8 +303,31 @@ int phc_ctx_autotune(phc_ctx *ctx, const phc_system *sys,
                      const phc_homotopy *hom, const phc_solve_options *opts,
                      int64_t npaths, const double *start, int32_t *choice,
                      double *best_ms);
+/* Autotuner of a batch (phc_track_batch): for every lanes class of the
+ * batch, times each launch shape the class allows (lane mapping grid, row
+ * or split; 64-, 128- or 256-thread workgroups; program staged in LDS when
+ * the class holds one system) on a sample -- the first paths of every
+ * system of the class, about 16 k paths in all -- with launches bounded to
+ * 64 linearization units per path slot, and keeps the shape of highest
+ * unit throughput for later batches of the same class shape (lanes class,
+ * system count, total terms, log2 of the total path count) while the
+ * mapping is 0. choices (room for 3 x 4 values) gets per class
+ * {G, mapping (1 row, 2 split, 3 grid), threads, staged}; *nclasses the
+ * class count. Results never depend on the choice. No counterpart in the
+ * reference (its tracker is serial). */
+int phc_ctx_autotune_batch(phc_ctx *ctx, int32_t nsys, const phc_system *sys,
+                           const phc_homotopy *hom,
+                           const phc_solve_options *opts,
+                           const int64_t *npaths, const double *const *start,
+                           int32_t *choices, int32_t *nclasses);
 /* Forgets every autotuned choice. */
 int phc_ctx_clear_tuning(phc_ctx *ctx);
+/* Launch shapes of the lanes classes of the last phc_track_paths /
+ * phc_track_batch / phc_stage_run call: for class c < max_classes,
+ * info[8c .. 8c+7] = {G, workgroups, threads, LDS bytes, staged, mapping
+ * (0 row, 1 split, 2 grid), lanes per path, systems}. Returns the class
+ * count. No counterpart in the reference. */
+int phc_ctx_last_plan(const phc_ctx *ctx, int32_t max_classes, int32_t *info);
 
 /* ---- batched multi-system tracker ------------------------------------- */
 /* Tracks the paths of nsys independent solves: one persistent launch per

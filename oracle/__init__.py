@@ -21,6 +21,27 @@ REF_LIB = os.path.join(HERE, "_ref", "libphc_ref.so")
 PORT_LIB = os.path.join(HERE, "libphc_oracle.so")
 
 
+def build(reference=None):
+    """Builds the checkers with oracle/Makefile: the C restatement always;
+    the compiled reference, and the reference's own test programs linked
+    against the product library, when the reference's sources are present
+    (in the build container; the GPU hosts use the prebuilt files)."""
+    import subprocess
+    ref = reference or os.environ.get("PHC_REFERENCE", "/root/reference")
+    targets = ["port"]
+    if os.path.isdir(os.path.join(ref, "src")):
+        targets.append("ref")
+        prod = os.path.join(os.path.dirname(HERE), "polyhomocont_amd", "lib",
+                            "libpolyhomocont_amd.so")
+        if os.path.exists(prod):
+            targets.append("droptests")
+    cmd = ["make", "-s", "-C", HERE, f"REF={ref}", *targets]
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=1200)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+
+
 class _Oracle:
     def __init__(self, path, prefix):
         if not os.path.exists(path):

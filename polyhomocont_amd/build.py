@@ -18,7 +18,6 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 FSRC = os.path.join(HERE, "fortran")
 LIB = os.path.join(HERE, "lib")
@@ -88,22 +87,6 @@ def build_fortran():
     return out
 
 
-def build_oracle():
-    """The checkers (test infrastructure): the C restatement always, the
-    compiled reference when its sources are present."""
-    odir = os.path.join(ROOT, "oracle")
-    targets = []
-    if os.path.exists(os.path.join(odir, "phc_oracle.c")):
-        targets.append("port")
-    ref = os.environ.get("PHC_REFERENCE", "/root/reference")
-    if os.path.isdir(os.path.join(ref, "src")):
-        targets.append("ref")
-        if os.path.exists(os.path.join(LIB, "libpolyhomocont_amd.so")):
-            targets.append("droptests")
-    if targets:
-        _run(["make", "-s", "-C", odir, f"REF={ref}", *targets])
-
-
 def build_wire():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libphc_wire.so")
@@ -115,10 +98,10 @@ def build_wire():
 
 
 def build_all():
+    """The product: HIP kernels, frame codec, Fortran host library."""
     build_hip()
     build_wire()
     build_fortran()
-    build_oracle()
 
 
 if __name__ == "__main__":

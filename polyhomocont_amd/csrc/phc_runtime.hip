@@ -41,11 +41,13 @@ struct phc_ctx {
   int mapping = 0;
   int staging = -1;    // program in LDS: -1 when it fits, 0 never
   size_t lds_limit = 64 * 1024;  // largest LDS of one workgroup
-  // autotuned launch choices of single-request runs, keyed by the request's
-  // shape (G, N, terms, log2 of the path count): mapping 1-3, workgroup
-  // threads, staging (phc_ctx_autotune)
+  // autotuned launch choices, keyed by the shape of a lanes class (see
+  // tune_key): mapping 1-3, workgroup threads, staging (phc_ctx_autotune,
+  // phc_ctx_autotune_batch)
   std::map<std::array<int64_t, 4>, std::array<int, 3>> tuned;
   unsigned long long counters[4] = {0, 0, 0, 0};  // of the last run
+  // launch shape of every lanes class of the last run (phc_ctx_last_plan)
+  std::vector<std::array<int32_t, 8>> last_plan;
   std::mutex mu;
   char err[512] = {0};
 };
@@ -486,6 +488,7 @@ struct phc_staged {
   Arena arena;
   std::vector<std::unique_ptr<HostReq>> reqs;
   std::vector<ClassRun> classes;
+  int32_t max_units = 0;  // units per path slot and launch (0: unbounded)
   ~phc_staged() {
     for (ClassRun& cr : classes) {
       if (cr.ev0) (void)hipEventDestroy(cr.ev0);
@@ -502,26 +505,42 @@ void* arena_take(Arena& a, size_t bytes) {
   return p;
 }
 
-// Key of the autotuner's cache: lanes class, unknowns, terms and the
-// binade of the path count.
-std::array<int64_t, 4> tune_key(const HostReq& h) {
+int64_t binade(int64_t v) {
   int64_t b = 0;
-  while ((int64_t(1) << (b + 1)) <= std::max<int64_t>(h.npaths, 1)) ++b;
-  return {h.G, h.r.N, h.r.nterms, b};
+  while ((int64_t(1) << (b + 1)) <= std::max<int64_t>(v, 1)) ++b;
+  return b;
 }
 
-// Lane mapping, launch shape and LDS of one lanes class. A single-request
-// class takes the autotuner's cached choice when there is one. Otherwise
-// (mapping 0): the grid unit when every request fits its tables, else one
-// path per wavefront with split terms when one path per group would leave
-// most SIMDs idle, else one path per group. Mappings 1-3 force row, split
-// or grid where the class allows it.
+// Key of the autotuner's cache. A class of one request: lanes class,
+// unknowns, terms and the binade of the path count. A class of several
+// requests (a batch): lanes class, minus the request count, the terms of
+// all its requests and the binade of their path count.
+std::array<int64_t, 4> tune_key(const HostReq& h) {
+  return {h.G, h.r.N, h.r.nterms, binade(h.npaths)};
+}
+std::array<int64_t, 4> tune_key(const std::vector<HostReq*>& reqs) {
+  if (reqs.size() == 1) return tune_key(*reqs[0]);
+  int64_t paths = 0, terms = 0;
+  for (const HostReq* h : reqs) {
+    paths += h->npaths;
+    terms += h->r.nterms;
+  }
+  return {reqs[0]->G, -(int64_t)reqs.size(), terms, binade(paths)};
+}
+
+// Lane mapping, launch shape and LDS of one lanes class. A class takes the
+// autotuner's cached choice for its shape when there is one (tune_key:
+// single requests and batch classes alike). Otherwise (mapping 0): the
+// grid unit when every request fits its tables, else one path per
+// wavefront with split terms when one path per group would leave most
+// SIMDs idle, else one path per group. Mappings 1-3 force row, split or
+// grid where the class allows it.
 void plan_class(phc_ctx* ctx, ClassRun& cr) {
   const int G = cr.G;
   int mapping = ctx->mapping, wg_threads = ctx->wg_threads,
       staging = ctx->staging;
-  if (mapping == 0 && cr.reqs.size() == 1) {
-    auto it = ctx->tuned.find(tune_key(*cr.reqs[0]));
+  if (mapping == 0) {
+    auto it = ctx->tuned.find(tune_key(cr.reqs));
     if (it != ctx->tuned.end()) {
       mapping = it->second[0];
       wg_threads = it->second[1];
@@ -732,6 +751,11 @@ int run_classes(phc_ctx* ctx, phc_staged* st) {
     if (!cr.ev0) HIPCHK(ctx, hipEventCreate(&cr.ev0));
     if (!cr.ev1) HIPCHK(ctx, hipEventCreate(&cr.ev1));
   }
+  ctx->last_plan.clear();
+  for (ClassRun& cr : st->classes)
+    ctx->last_plan.push_back({cr.G, cr.blocks, cr.threads, (int32_t)cr.lds,
+                              cr.staged ? 1 : 0, cr.mode, cr.lp,
+                              (int32_t)cr.reqs.size()});
   for (ClassRun& cr : st->classes)
     HIPCHK(ctx, hipEventRecord(cr.ev0, ctx->stream));
   for (size_t i = 0; i < nc; ++i) {
@@ -749,7 +773,7 @@ int run_classes(phc_ctx* ctx, phc_staged* st) {
     L.total = cr.total;
     L.slots = nullptr;
     L.resume = 0;
-    L.max_units = 0;
+    L.max_units = st->max_units;
     L.active = cr.d_active;
     L.counters = cr.d_counters;
     HIPCHK(ctx, launch_g(cr.G, L, cr.staged, cr.blocks, cr.threads, cr.lds,
@@ -1129,6 +1153,129 @@ int phc_ctx_autotune(phc_ctx* ctx, const phc_system* sys,
   choice[2] = pick[2] == 0 ? 0 : 1;
   if (best_ms) *best_ms = best;
   return PHC_OK;
+}
+
+int phc_ctx_autotune_batch(phc_ctx* ctx, int32_t nsys, const phc_system* sys,
+                           const phc_homotopy* hom,
+                           const phc_solve_options* opts,
+                           const int64_t* npaths, const double* const* start,
+                           int32_t* choices, int32_t* nclasses) {
+  if (!ctx || !choices || !nclasses) return PHC_ERR_INVALID_INPUT;
+  *nclasses = 0;
+  if (nsys < 0 || (nsys > 0 && (!sys || !hom || !opts || !npaths || !start)))
+    return fail(ctx, PHC_ERR_INVALID_INPUT, "null batch input");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // the batch's lanes classes with their full path counts (the cache key)
+  std::vector<std::unique_ptr<HostReq>> full;
+  std::map<int, std::vector<int>> byG;  // G -> systems
+  for (int s = 0; s < nsys; ++s) {
+    if (npaths[s] == 0) continue;
+    if (!start[s]) return fail(ctx, PHC_ERR_INVALID_INPUT, "autotune needs start points");
+    std::unique_ptr<HostReq> h(new HostReq);
+    int rc = validate_and_pack(ctx, &sys[s], &hom[s], &opts[s], npaths[s], *h);
+    if (rc) return rc;
+    byG[h->G].push_back(s);
+    full.push_back(std::move(h));
+  }
+  struct Restore {  // the context's forced choices, whatever happens below
+    phc_ctx* c;
+    int m, w, st;
+    ~Restore() {
+      c->mapping = m;
+      c->wg_threads = w;
+      c->staging = st;
+    }
+  } restore{ctx, ctx->mapping, ctx->wg_threads, ctx->staging};
+  int nc = 0;
+  for (auto& kv : byG) {
+    const int G = kv.first;
+    std::vector<HostReq*> members;
+    for (auto& up : full)
+      if (up->G == G) members.push_back(up.get());
+    const std::array<int64_t, 4> key = tune_key(members);
+    // the sample: the first paths of every system of the class, about
+    // 16 k paths in all
+    const int64_t cap =
+        std::max<int64_t>(16, 16384 / (int64_t)kv.second.size());
+    bool can_split = G <= 8;
+    for (HostReq* h : members) can_split = can_split && h->r.N <= 64 / G;
+    std::vector<std::array<int, 4>> seen;  // mode, threads, staged, blocks
+    double best = -1.0;
+    std::array<int, 3> pick = {0, 0, -1};
+    for (int m : {3, 1, 2}) {
+      if (m == 2 && !can_split) continue;
+      for (int wg : {64, 128, 256})
+        for (int stg : {-1, 0}) {
+          ctx->mapping = m;
+          ctx->wg_threads = wg;
+          ctx->staging = stg;
+          std::unique_ptr<phc_staged> st(new phc_staged);
+          bool packed = true;
+          for (int s : kv.second) {
+            std::unique_ptr<HostReq> h(new HostReq);
+            if (validate_and_pack(ctx, &sys[s], &hom[s], &opts[s],
+                                  std::min<int64_t>(npaths[s], cap), *h)) {
+              packed = false;
+              break;
+            }
+            h->start = start[s];
+            st->reqs.push_back(std::move(h));
+          }
+          if (!packed || stage_requests(ctx, st.get()) != PHC_OK ||
+              st->classes.size() != 1)
+            continue;
+          const ClassRun& cr = st->classes[0];
+          const std::array<int, 4> id = {cr.mode, cr.threads, cr.staged ? 1 : 0,
+                                         cr.blocks};
+          bool dup = false;
+          for (const auto& x : seen) dup = dup || x == id;
+          if (dup) continue;
+          seen.push_back(id);
+          // bounded launches: every path slot runs at most this many units,
+          // so the score is the class's unit throughput, not its longest path
+          st->max_units = 64;
+          unsigned long long c[4] = {0, 0, 0, 0};
+          float ms = 0.f;
+          if (run_classes(ctx, st.get()) != PHC_OK ||  // code load, caches
+              hipStreamSynchronize(ctx->stream) != hipSuccess ||
+              run_classes(ctx, st.get()) != PHC_OK ||
+              hipMemcpyAsync(c, st->classes[0].d_counters, sizeof(c),
+                             hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+              hipStreamSynchronize(ctx->stream) != hipSuccess ||
+              hipEventElapsedTime(&ms, st->classes[0].ev0,
+                                  st->classes[0].ev1) != hipSuccess)
+            continue;
+          const double score = (double)c[1] / std::max((double)ms, 1e-6);
+          if (score > best) {
+            best = score;
+            const int mapping =
+                cr.mode == MODE_GRID ? 3 : (cr.mode == MODE_SPLIT ? 2 : 1);
+            pick = {mapping, cr.threads, cr.staged ? -1 : 0};
+          }
+        }
+    }
+    if (best < 0.0)
+      return fail(ctx, PHC_ERR_DEVICE,
+                  "no launch shape of lanes class %d could be timed", G);
+    ctx->tuned[key] = pick;
+    choices[4 * nc + 0] = G;
+    choices[4 * nc + 1] = pick[0];
+    choices[4 * nc + 2] = pick[1];
+    choices[4 * nc + 3] = pick[2] == 0 ? 0 : 1;
+    ++nc;
+  }
+  *nclasses = nc;
+  return PHC_OK;
+}
+
+int phc_ctx_last_plan(const phc_ctx* ctx, int32_t max_classes, int32_t* info) {
+  if (!ctx || max_classes < 0 || (max_classes > 0 && !info))
+    return PHC_ERR_INVALID_INPUT;
+  const int n = (int)ctx->last_plan.size();
+  for (int c = 0; c < n && c < max_classes; ++c)
+    for (int k = 0; k < 8; ++k) info[8 * c + k] = ctx->last_plan[c][k];
+  return n;
 }
 
 int phc_ctx_clear_tuning(phc_ctx* ctx) {

@@ -47,6 +47,8 @@ def lib():
         L.phc_ctx_set_mapping.argtypes = [P, C.c_int, C.c_int]
         L.phc_ctx_autotune.argtypes = [P, P, P, P, i64, P, P, P]
         L.phc_ctx_clear_tuning.argtypes = [P]
+        L.phc_ctx_autotune_batch.argtypes = [P, i32, P, P, P, P, P, P, P]
+        L.phc_ctx_last_plan.argtypes = [P, i32, P]
         L.phc_track_paths.argtypes = [P, P, P, P, i64, P, P]
         L.phc_stage_create.argtypes = [P, P, P, P, i64, P, C.POINTER(P)]
         L.phc_stage_run.argtypes = [P, P]
@@ -155,8 +157,8 @@ class Context:
         self._check(rc, "phc_track_paths")
         return out
 
-    def track_batch(self, items):
-        """items: list of (packed, hom, opts, start); returns EndpointArrays."""
+    @staticmethod
+    def _batch_args(items):
         n = len(items)
         systems = (abi.System * n)(*[it[0].c for it in items])
         homs = (abi.Homotopy * n)(*[it[1].c for it in items])
@@ -164,6 +166,11 @@ class Context:
         starts = [np.ascontiguousarray(it[3], np.complex128) for it in items]
         npaths = np.array([s.shape[0] for s in starts], np.int64)
         sp = (C.c_void_p * n)(*[s.ctypes.data for s in starts])
+        return n, systems, homs, opts, starts, npaths, sp
+
+    def track_batch(self, items):
+        """items: list of (packed, hom, opts, start); returns EndpointArrays."""
+        n, systems, homs, opts, starts, npaths, sp = self._batch_args(items)
         outs = [abi.EndpointArrays(s.shape[0], it[0].n + 1)
                 for s, it in zip(starts, items)]
         oc = (abi.Endpoints * n)(*[o.c for o in outs])
@@ -171,6 +178,38 @@ class Context:
                                     npaths.ctypes.data, sp, oc)
         self._check(rc, "phc_track_batch")
         return outs
+
+    def autotune_batch(self, items):
+        """Times the launch shapes of every lanes class of this batch (on a
+        sample of its paths, bounded launches) and keeps the one of highest
+        unit throughput for later batches of the same class shapes; returns
+        the choice per class."""
+        n, systems, homs, opts, starts, npaths, sp = self._batch_args(items)
+        ch = np.zeros(12, np.int32)
+        nc = C.c_int32(0)
+        rc = self.L.phc_ctx_autotune_batch(self.h, n, systems, homs, opts,
+                                           npaths.ctypes.data, sp,
+                                           ch.ctypes.data, C.byref(nc))
+        self._check(rc, "phc_ctx_autotune_batch")
+        names = {1: "row", 2: "split", 3: "grid"}
+        return [dict(G=int(ch[4 * c]), mapping=names[int(ch[4 * c + 1])],
+                     threads=int(ch[4 * c + 2]), staged=bool(ch[4 * c + 3]))
+                for c in range(nc.value)]
+
+    def last_plan(self):
+        """Launch shape of every lanes class of the last tracking call."""
+        info = np.zeros(8 * 8, np.int32)
+        n = self.L.phc_ctx_last_plan(self.h, 8, info.ctypes.data)
+        if n < 0:
+            self._check(n, "phc_ctx_last_plan")
+        names = {0: "row", 1: "split", 2: "grid"}
+        return [dict(G=int(info[8 * c]), blocks=int(info[8 * c + 1]),
+                     threads=int(info[8 * c + 2]), lds=int(info[8 * c + 3]),
+                     staged=bool(info[8 * c + 4]),
+                     mapping=names[int(info[8 * c + 5])],
+                     lanes_per_path=int(info[8 * c + 6]),
+                     systems=int(info[8 * c + 7]))
+                for c in range(min(n, 8))]
 
 
 class Staged:

@@ -148,6 +148,43 @@ def test_batch_equals_separate_calls(ctx, host):
         assert_bit_exact(o, s, len(it[3]))
 
 
+def test_batch_autotune_is_used_and_bit_exact(ctx, host):
+    """The batch autotuner picks a launch shape per lanes class; the next
+    batch of the same class shapes launches with those shapes, and every
+    system's records equal its own phc_track_paths call bit for bit."""
+    # lanes classes 4 (n = 3), 8 (n = 4..7) and 16 (n = 8); the first 128
+    # paths of each (at most a few hundred steps per path)
+    pool = W.batch_systems(93)
+    systems = [pool[i] for i in (41, 42, 43, 44, 16, 17, 0, 90, 91, 92,
+                                 68, 69, 70)]
+    opts = abi.default_options()
+    items, single = [], []
+    for usys in systems:
+        g, b, a, ns = host.homotopy(usys, opts.seed)
+        start = host.start_points(usys, opts.seed, 0, min(ns, 128))
+        item = (host.pack(usys), abi.HomotopyData(g, b, a), opts, start)
+        items.append(item)
+        single.append(ctx.track_paths(*item))
+    try:
+        choices = ctx.autotune_batch(items)
+        print(f"[gpu] batch autotune of {len(items)} systems: {choices}",
+              flush=True)
+        assert sorted(c["G"] for c in choices) == sorted(
+            {ctx.lanes_per_path(it[0].n + 1) for it in items})
+        outs = ctx.track_batch(items)
+        plan = {p["G"]: p for p in ctx.last_plan()}
+        for c in choices:
+            p = plan[c["G"]]
+            assert p["mapping"] == c["mapping"], (c, p)
+            assert p["threads"] == c["threads"], (c, p)
+            assert p["systems"] == sum(
+                ctx.lanes_per_path(it[0].n + 1) == c["G"] for it in items)
+        for o, s, it in zip(outs, single, items):
+            assert_bit_exact(o, s, len(it[3]))
+    finally:
+        ctx.clear_tuning()
+
+
 @pytest.mark.parametrize("mapping", [1, 2, 3])
 def test_lane_mappings_bit_exact(ctx, mapping):
     """One path per lane group, one path per wavefront (terms split across
