@@ -97,9 +97,12 @@ class Engine:
     serve() runs a stream and returns per-request results and latencies."""
 
     def __init__(self, ctx, H, workers=4, host_threads=4, resident=16384,
-                 budget=2 << 30, units=UNITS_PER_LAUNCH):
+                 budget=2 << 30, units=UNITS_PER_LAUNCH,
+                 submits=SUBMITS_PER_CALL, prep_wait_us=500):
         from polyhomocont_amd import device
         self.H = H
+        self.submits = submits
+        self.prep_wait_us = prep_wait_us
         self.srv = device.Server(ctx, max_resident_paths=resident,
                                  hbm_budget_bytes=budget,
                                  units_per_launch=units)
@@ -126,7 +129,7 @@ class Engine:
                 nxt += 1
             # a bounded number of submissions between progress calls, so
             # that rounds that end meanwhile are restarted promptly
-            ready = [i for i, f in prep.items() if f.done()][:SUBMITS_PER_CALL]
+            ready = [i for i, f in prep.items() if f.done()][:self.submits]
             check = False
             for i in ready:
                 req = prep.pop(i).result()
@@ -135,8 +138,12 @@ class Engine:
                 inflight[t] = (i, out, req[0])
                 check = check or out.npaths == 0
             if inflight:
-                # tickets are polled only after a call that finished some
-                check = self.srv.progress() > 0 or check
+                # tickets are polled only after a call that finished some;
+                # while requests are still being prepared the call waits
+                # only briefly for a round to end, so that they are
+                # submitted as they become ready
+                check = self.srv.progress(
+                    self.prep_wait_us if prep else -1) > 0 or check
                 for t in [t for t in inflight if check and self.srv.done(t)]:
                     i, out, packed = inflight.pop(t)
                     self.srv.release(t)
@@ -207,7 +214,10 @@ def serving_section(ctx, H, args, out):
     stream = workloads.serving_stream(args.serving_requests)
     paths = sum(u.bezout for u in stream)
     log(f"serving stream: {len(stream)} requests, {paths} paths")
-    eng = Engine(ctx, H)
+    eng = Engine(ctx, H, workers=args.host_workers,
+                 resident=args.resident_slots, units=args.units_per_launch,
+                 submits=args.submits_per_call,
+                 prep_wait_us=args.prep_wait_us)
     for w in range(args.warmup):
         eng.serve(stream, opts)
         log(f"warmup pass {w + 1}/{args.warmup}")
@@ -251,8 +261,9 @@ def serving_section(ctx, H, args, out):
                     f"{paths} paths; continuous-batching engine, host "
                     "classification included",
         "requests": len(stream), "paths": paths,
-        "units_per_launch": UNITS_PER_LAUNCH, "resident_path_slots": 16384,
-        "host_pool_threads": 4}
+        "units_per_launch": args.units_per_launch,
+        "resident_path_slots": args.resident_slots,
+        "host_pool_workers": args.host_workers}
     out["roofline"] = {
         "bound": "fp64-valu",
         "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
@@ -272,7 +283,8 @@ def serving_section(ctx, H, args, out):
          "launches": int(x["launches"]), "units": int(x["units"]),
          "tflops": round(fl[i] * args.steps / max(x["kernel_ms"], 1e-9) / 1e9, 4),
          "slot_time_utilisation": round(
-             x["units"] / max(x["slots"] * x["launches"] * UNITS_PER_LAUNCH, 1), 4),
+             x["units"] / max(x["slots"] * x["launches"] * args.units_per_launch,
+                              1), 4),
          "lane_occupancy": round(x["units"] / x["unit_slots"], 4)
          if x["unit_slots"] else None}
         for i, x in enumerate(delta)]
@@ -573,10 +585,27 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="systems of the batched multi-system launch (0: skip)")
+    ap.add_argument("--units-per-launch", type=int, default=UNITS_PER_LAUNCH,
+                    help="serving engine: linearizations per path slot and "
+                         "launch")
+    ap.add_argument("--submits-per-call", type=int, default=SUBMITS_PER_CALL,
+                    help="serving engine: requests submitted between two "
+                         "progress calls")
+    ap.add_argument("--prep-wait-us", type=int, default=500,
+                    help="serving engine: longest wait of a progress call "
+                         "while requests are being prepared (-1: wait for "
+                         "a round to end)")
+    ap.add_argument("--resident-slots", type=int, default=16384,
+                    help="serving engine: resident path slots")
+    ap.add_argument("--host-workers", type=int, default=4,
+                    help="serving engine: host pool workers (each with 4 "
+                         "threads) for constants, start points and "
+                         "classification")
     ap.add_argument("--no-plain", action="store_true",
                     help="serving: engine passes only (no plain per-solve "
                          "path, no latency runs); for profiling the engine")
-    ap.add_argument("--only", choices=("serving", "c2", "batch", "c3"),
+    ap.add_argument("--only", choices=("serving", "c2", "batch", "c3",
+                                       "hbm-copy"),
                     help="run one section alone (c3, the 5040-path end game "
                          "stress solve, runs only when named here)")
     args = ap.parse_args()
@@ -591,6 +620,19 @@ def main():
     # then share (a runtime loaded before torch's leaves torch without GPUs)
     import torch
     torch.cuda.init()
+    if args.only == "hbm-copy":
+        # calibration of the HBM byte counters: a plain device copy of
+        # 256 MiB (read 256 MiB, write 256 MiB per launch), 5 launches
+        n = 256 << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"hbm_copy_bytes_read": n, "hbm_copy_bytes_written": n,
+                          "launches": 5}), flush=True)
+        return
     from polyhomocont_amd import device, host
     H = host.library()
     ctx = device.Context(0)

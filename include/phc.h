@@ -368,6 +368,12 @@ int phc_server_submit(phc_server *srv, const phc_system *sys,
  * end (without holding the context). Returns the number of requests
  * completed by this call. */
 int phc_server_progress(phc_server *srv);
+/* phc_server_progress with the wait bounded: when no round has ended and
+ * none could start, waits at most max_wait_us microseconds for one to end
+ * (0: returns at once; negative: no bound, as phc_server_progress). For a
+ * caller with host work of its own in hand, such as requests to submit.
+ * No counterpart in the reference. */
+int phc_server_progress_for(phc_server *srv, int64_t max_wait_us);
 /* PHC_OK when the request's records are in its `out`, PHC_ERR_PENDING if
  * not yet; phc_server_wait blocks (driving progress) until done. */
 int phc_server_poll(phc_server *srv, uint64_t ticket);

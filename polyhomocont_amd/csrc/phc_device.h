@@ -65,6 +65,7 @@ struct alignas(16) Req {
                      // 0-7), of a job's prefix (8-15) and suffix (16-23),
                      // product length of the short job layout (24-27)
   int32_t vstride;   // grid unit: V entries per equation
+  int32_t dbits;     // bits of the largest deg[i] - 1 (start-system powers)
   int32_t first[PHC_MAX_UNKNOWNS + 1];
   int32_t deg[PHC_MAX_UNKNOWNS];
   cx gamma;
@@ -487,8 +488,8 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
     const cx omt = mk(1.0 - t.re, 0.0 - t.im);
 #pragma unroll
     for (int j = 0; j < G; ++j) hz[j] = cmul(omt, hz[j]);
-    const cx zd1 = zpowi(zq, d - 1);
-    const cx wd1 = zpowi(w, d - 1);
+    cx zd1, wd1;
+    zpowi_pair(zq, w, d - 1, R.dbits, zd1, wd1);
     const cx bi = R.b[lane];
     const cx g = csub(cmul(zd1, zq), cmul(cmul(bi, wd1), w));
     h = cadd(cmul(gt, g), cmul(omt, f));
@@ -508,7 +509,9 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
 #pragma unroll
     for (int j = 0; j < G; ++j) {
       const cx pj = gshfl_c<G>(pz, j);
-      if (j < N) ssum = cadd(ssum, pj);
+      const cx sj = cadd(ssum, pj);
+      ssum.re = j < N ? sj.re : ssum.re;  // selects: no branch per term
+      ssum.im = j < N ? sj.im : ssum.im;
     }
     if (lane == n) {
       h = mk(ssum.re - 1.0, ssum.im - 0.0);

@@ -187,6 +187,36 @@ PHC_HD cx zpowi(cx base, int e) {
   return acc;
 }
 
+// z^e and w^e for 0 <= e < 2^nbits with exactly the products of zpowi
+// (the squarings of the base, and the product of the squares of the set
+// bits taken from the lowest set bit up, the accumulated product on the
+// left), but without branches: every lane runs the nbits - 1 squarings and
+// the nbits products and keeps its own by selects, so lanes with different
+// exponents do not serialize. nbits is the same for all lanes; e = 0 gives
+// (1, 0). Squarings past the highest set bit are formed and not used.
+PHC_HD void zpowi_pair(cx z, cx w, int e, int nbits, cx& ze, cx& we) {
+  double zr = 1.0, zi = 0.0, wr = 1.0, wi = 0.0;  // the products so far
+  bool have = false;
+  for (int b = 0; b < nbits; ++b) {
+    if (b) {
+      z = cmul(z, z);
+      w = cmul(w, w);
+    }
+    const bool bit = ((e >> b) & 1) != 0;
+    const cx pz = cmul(mk(zr, zi), z), pw = cmul(mk(wr, wi), w);
+    // selects on the doubles themselves (a select between whole structs
+    // can turn into a stack slot picked by address)
+    const bool mul = bit && have, first = bit && !have;
+    zr = mul ? pz.re : (first ? z.re : zr);
+    zi = mul ? pz.im : (first ? z.im : zi);
+    wr = mul ? pw.re : (first ? w.re : wr);
+    wi = mul ? pw.im : (first ? w.im : wi);
+    have = have || bit;
+  }
+  ze = mk(zr, zi);
+  we = mk(wr, wi);
+}
+
 // ---- sin and cos as glibc's sincos computes them ------------------------
 // The reference's arc angles (src/tracker.f90:159) reach glibc 2.35's
 // sincos (flang -O2 merges the cos/sin pair into one call). Its dbl-64

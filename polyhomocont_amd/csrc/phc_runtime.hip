@@ -154,6 +154,9 @@ int validate_and_pack(phc_ctx* ctx, const phc_system* sys,
     (void)tot;
   }
   r.maxexp = std::max(maxexp, 1);
+  int dmax = 0;
+  for (int i = 0; i < n; ++i) dmax = std::max(dmax, r.deg[i] - 1);
+  while (dmax >> r.dbits) ++r.dbits;
   // Nonzero-exponent mask of each term in e[3] bits 16..31. Exponent bytes
   // of unknowns 0..7 live in e[0..1], so e[3] is free only when N <= 8; the
   // decode of wider groups ignores bytes at j >= N.

@@ -161,6 +161,17 @@ template <int LP>
 PHC_D cx pshfl_c(cx v, int src) {
   return mk(__shfl(v.re, src, LP), __shfl(v.im, src, LP));
 }
+// A divisor prepared by lane `src` (cdiv_prep of its own entry).
+template <int LP>
+PHC_D cdiv_den pbcast_den(const cdiv_den& v, int src) {
+  cdiv_den r;
+  r.c = pbcast_d<LP>(v.c, src);
+  r.d = pbcast_d<LP>(v.d, src);
+  r.denom = pbcast_d<LP>(v.denom, src);
+  r.logbw = pbcast_d<LP>(v.logbw, src);
+  r.ilogbw = pbcast_i<LP>(v.ilogbw, src);
+  return r;
+}
 
 // Slots e of a lane that hold a column of an N-unknown system: columns
 // r + R e < N for some group r.
@@ -204,12 +215,26 @@ PHC_D void term_values(const Req& R, WP wt,
                        const cx* __restrict__ T, cx* __restrict__ V, int q) {
   const int nt = R.nterms;
   int k = q;
-  for (; k + LP < nt; k += 2 * LP) {
-    const WTerm w0 = ld(wt, k), w1 = ld(wt, k + LP);
-    const cx v0 = term_value<FV>(w0, T);
-    const cx v1 = term_value<FV>(w1, T);
-    V[w0.slot] = v0;
-    V[w1.slot] = v1;
+  if (k + LP < nt) {
+    // the next turn's two terms are loaded while this turn's compute
+    WTerm w0 = ld(wt, k), w1 = ld(wt, k + LP);
+    for (;;) {
+      const int kn = k + 2 * LP;
+      const bool more = kn + LP < nt;
+      WTerm n0 = w0, n1 = w1;
+      if (more) {
+        n0 = ld(wt, kn);
+        n1 = ld(wt, kn + LP);
+      }
+      const cx v0 = term_value<FV>(w0, T);
+      const cx v1 = term_value<FV>(w1, T);
+      V[w0.slot] = v0;
+      V[w1.slot] = v1;
+      k = kn;
+      if (!more) break;
+      w0 = n0;
+      w1 = n1;
+    }
   }
   if (k < nt) {
     const WTerm w0 = ld(wt, k);
@@ -524,6 +549,9 @@ PHC_D int lu_slots(int N, cx (&a)[GridShape<G>::E],
         const int k = ke * RG + kr;
         if (!ok || k >= N) break;
         const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
+        // every lane prepares its own entry as a divisor while the pivot is
+        // searched; the winner's is broadcast (cdiv_prep of the pivot)
+        const cdiv_den mine = cdiv_prep(a[ke]);
         double bv;
         int bp, bl;
         pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, lane,
@@ -533,7 +561,7 @@ PHC_D int lu_slots(int N, cx (&a)[GridShape<G>::E],
           break;
         }
         plb |= (uint64_t)bl << (4 * k);
-        const cx piv = pbcast_c<LP>(a[ke], kr * G + bl);
+        const cdiv_den pd = pbcast_den<LP>(mine, kr * G + bl);
         // slots left of ke hold finished columns (col[e] < k)
         cx pr[E];
 #pragma unroll
@@ -542,7 +570,7 @@ PHC_D int lu_slots(int N, cx (&a)[GridShape<G>::E],
         if (pos == k) np = bp;
         if (lane == bl) np = k;
         pos = np;
-        const cx li = cdiv(aik, piv);
+        const cx li = cdiv_apply(aik, pd);
         if (in[ke] && rep == kr && pos > k) a[ke] = li;
 #pragma unroll
         for (int e = ke; e < EU; ++e) {
@@ -573,6 +601,7 @@ PHC_D int lu_one_slot_exact(int N, cx& a, const bool in, const int col,
     // column k of this lane's row, fetched while the pivot is searched:
     // every group forms its row's multiplier itself
     const cx aik = pshfl_c<LP>(a, k * G + lane);
+    const cdiv_den mine = cdiv_prep(a);  // the winner's is the pivot's
     double bv;
     int bp, bl;
     pivot_search<G, LP>(in && rep == k && pos >= k, a, pos, lane, k * G, bv,
@@ -582,13 +611,13 @@ PHC_D int lu_one_slot_exact(int N, cx& a, const bool in, const int col,
       break;
     }
     plb |= (uint64_t)bl << (4 * k);
-    const cx piv = pbcast_c<LP>(a, k * G + bl);
+    const cdiv_den pd = pbcast_den<LP>(mine, k * G + bl);
     const cx pr = pshfl_c<LP>(a, rep * G + bl);
     int np = pos;
     if (pos == k) np = bp;
     if (lane == bl) np = k;
     pos = np;
-    const cx li = cdiv(aik, piv);
+    const cx li = cdiv_apply(aik, pd);
     if (in && rep == k && pos > k) a = li;
     const cx upd = csub(a, cmul(li, pr));
     if (in && col > k && pos > k) a = upd;
@@ -625,6 +654,7 @@ PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
     const cx aik = T[k * G + lane];
     const int gbase = k * G;
     const bool cand = in && rep == k && pos >= k;
+    const cdiv_den mine = cdiv_prep(a);  // the winner's is the pivot's
     const double sq = a.re * a.re + a.im * a.im;
     const double sv = pbcast_d<LP>(gmax<G>(cand ? sq : -1.0), gbase);
     const unsigned long long hit = (__ballot(cand && sq == sv) >> shift) & pmask;
@@ -638,13 +668,13 @@ PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
     const int bl = wl - gbase;
     const int bp = pbcast_i<LP>(pos, wl);
     plb |= (uint64_t)(bl & 15) << (4 * k);
-    const cx piv = T[k * G + bl];
+    const cdiv_den pd = pbcast_den<LP>(mine, wl);
     const cx pr = T[rep * G + bl];
     int np = pos;
     if (pos == k) np = bp;
     if (lane == bl) np = k;
     pos = np;
-    const cx li = cdiv(aik, piv);
+    const cx li = cdiv_apply(aik, pd);
     if (in && rep == k && pos > k) a = li;
     const cx upd = csub(a, cmul(li, pr));
     if (in && col > k && pos > k) a = upd;
@@ -680,6 +710,7 @@ PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
         const cx aik = T[k * G + lane];
         const int gbase = kr * G;
         const bool cand = in[ke] && rep == kr && pos >= k;
+        const cdiv_den mine = cdiv_prep(a[ke]);  // the winner's: the pivot's
         const double sq = a[ke].re * a[ke].re + a[ke].im * a[ke].im;
         const double sv = pbcast_d<LP>(gmax<G>(cand ? sq : -1.0), gbase);
         const unsigned long long hit = __ballot(cand && sq == sv);
@@ -690,12 +721,12 @@ PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
         const int bl = wl - gbase;
         const int bp = pbcast_i<LP>(pos, wl);
         plb |= (uint64_t)(bl & 15) << (4 * k);
-        const cx piv = T[k * G + bl];
+        const cdiv_den pd = pbcast_den<LP>(mine, wl);
         int np = pos;
         if (pos == k) np = bp;
         if (lane == bl) np = k;
         pos = np;
-        const cx li = cdiv(aik, piv);
+        const cx li = cdiv_apply(aik, pd);
         if (in[ke] && rep == kr && pos > k) a[ke] = li;
 #pragma unroll
         for (int e = ke; e < EU; ++e) {
@@ -874,19 +905,32 @@ PHC_D void unit_wave(const Req& R, WP wt,
     const int il = lane < n ? lane : 0;
     const int dl = R.deg[il];
     const cx zl = T[G + il];
-    const cx zdl = zpowi(zl, dl - 1);
-    const cx wdl = zpowi(w, dl - 1);
+    cx zdl, wdl;
+    zpowi_pair(zl, w, dl - 1, R.dbits, zdl, wdl);
     const cx bl = R.b[il];
     const cx diag_add = cmul(cscale((double)dl, gt), zdl);
     const cx wcol_sub = cmul(cscale((double)dl, cmul(gt, bl)), wdl);
-    // with several slots per lane, the F slots of a lane would each form
-    // their equation's start-system value; group 0's row lanes form it once
-    // and pass it through HT, which the F slots overwrite after reading
-    if constexpr (E > 1) {
-      if (rep == 0 && lane < n)
-        W.HT[lane] = csub(cmul(zdl, zl), cmul(cmul(bl, wdl), w));
-      wave_lds_fence();
+    // each equation's start-system value g_i = z_i^d - b_i w^d is formed
+    // once, by row lane i of group 0, and passed to the F slot through HT
+    // (which that slot overwrites after reading)
+    if (rep == 0 && lane < n)
+      W.HT[lane] = csub(cmul(zdl, zl), cmul(cmul(bl, wdl), w));
+    // the patch row's sum(a*z) in index order, unrolled so that its loads
+    // and products do not wait on the additions; every lane forms it and
+    // lane n keeps it
+    // (every load and product is formed, entries past N included -- the
+    // patch is zero there, the table row holds any finite or non-finite
+    // value -- and the sum keeps only the first N by selects, so that no
+    // branch holds the loads back)
+    cx psum = mk(0.0, 0.0);
+#pragma unroll
+    for (int jj = 0; jj < G; ++jj) {
+      const cx pj = cmul(R.patch[jj], T[G + jj]);
+      const cx sj = cadd(psum, pj);
+      psum.re = jj < N ? sj.re : psum.re;
+      psum.im = jj < N ? sj.im : psum.im;
     }
+    wave_lds_fence();
     static_for<E>([&](auto ec) {
       constexpr int e = decltype(ec)::value;
       const int j = col[e];
@@ -898,17 +942,7 @@ PHC_D void unit_wave(const Req& R, WP wt,
         if (j == n) a[e] = csub(a[e], wcol_sub);
       } else if (fl[e]) {
         const int i = j;
-        cx g;
-        if constexpr (E > 1) {
-          g = W.HT[i];
-        } else {
-          const int d = R.deg[i];
-          const cx zi = T[G + i];
-          const cx zd1 = zpowi(zi, d - 1);
-          const cx wd1 = zpowi(w, d - 1);
-          const cx bi = R.b[i];
-          g = csub(cmul(zd1, zi), cmul(cmul(bi, wd1), w));
-        }
+        const cx g = W.HT[i];
         const cx h = cadd(cmul(gt, g), cmul(omt, acc[e]));
         const cx ht = csub(cmul(R.gamma, g), acc[e]);
         W.H[i] = h;
@@ -917,10 +951,7 @@ PHC_D void unit_wave(const Req& R, WP wt,
       }
       if (lane == n && j < N) a[e] = R.patch[j];
       if (lane == n && j == n) {
-        cx ssum = mk(0.0, 0.0);
-        for (int jj = 0; jj < N; ++jj)
-          ssum = cadd(ssum, cmul(R.patch[jj], T[G + jj]));
-        const cx h = mk(ssum.re - 1.0, ssum.im - 0.0);
+        const cx h = mk(psum.re - 1.0, psum.im - 0.0);
         const cx ht = mk(0.0, 0.0);
         W.H[n] = h;
         W.HT[n] = ht;

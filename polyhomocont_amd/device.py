@@ -60,6 +60,7 @@ def lib():
         L.phc_server_create.argtypes = [P, P, C.POINTER(P)]
         L.phc_server_submit.argtypes = [P, P, P, P, i64, P, P, P]
         L.phc_server_progress.argtypes = [P]
+        L.phc_server_progress_for.argtypes = [P, i64]
         L.phc_server_poll.argtypes = [P, C.c_uint64]
         L.phc_server_wait.argtypes = [P, C.c_uint64]
         L.phc_server_release.argtypes = [P, C.c_uint64]
@@ -296,11 +297,12 @@ class Server:
         self._keep[t.value] = (packed, hom, opts, start, out)
         return t.value, out
 
-    def progress(self):
+    def progress(self, max_wait_us=-1):
         """Retires the classes whose round ended, admits and starts the
         next rounds; waits for a round to end only when none has and none
-        could start. Returns the number of requests finished."""
-        rc = self.ctx.L.phc_server_progress(self.h)
+        could start (at most max_wait_us microseconds when that is not
+        negative). Returns the number of requests finished."""
+        rc = self.ctx.L.phc_server_progress_for(self.h, max_wait_us)
         if rc < 0:
             self.ctx._check(rc, "phc_server_progress")
         return rc

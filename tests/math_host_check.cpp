@@ -99,6 +99,38 @@ int main(int argc, char** argv) {
   }
   printf("zpowi %ld %ld\n", cnt, bad);
 
+  // the branch-free pair of the kernels' start-system powers: exponents
+  // 0 .. 2^nbits - 1 under every nbits that covers them (a lane's exponent
+  // need not use the top bits); finite bases against the runtime routine,
+  // special bases against zpowi (the runtime's multiply recovers infinities
+  // from NaN products, C99 Annex G, which neither kernel routine does)
+  bad = cnt = 0;
+  for (long i = 0; i < n / 4; ++i) {
+    double re = wide(8), im = wide(8), wre = wide(8), wim = wide(8);
+    const int nbits = 1 + (int)(next_u64() % 6);
+    const int e = (int)(next_u64() % (uint64_t)(1 << nbits));
+    const int bits = nbits + (int)(next_u64() % 3);
+    cx pz, pw;
+    phc::zpowi_pair(phc::mk(re, im), phc::mk(wre, wim), e, bits, pz, pw);
+    cx wz = _FortranAzpowi(re, im, e), ww = _FortranAzpowi(wre, wim, e);
+    bad += !(same(pz.re, wz.re) && same(pz.im, wz.im));
+    bad += !(same(pw.re, ww.re) && same(pw.im, ww.im));
+    cnt += 2;
+  }
+  for (int i = 0; i < NSPECIAL; ++i)
+    for (int j = 0; j < NSPECIAL; ++j)
+      for (int e = 0; e < 16; ++e) {
+        const cx z = phc::mk(SPECIAL[i], SPECIAL[j]);
+        const cx w = phc::mk(SPECIAL[j], SPECIAL[(i + e) % NSPECIAL]);
+        cx pz, pw;
+        phc::zpowi_pair(z, w, e, 4, pz, pw);
+        const cx wz = phc::zpowi(z, e), ww = phc::zpowi(w, e);
+        bad += !(same(pz.re, wz.re) && same(pz.im, wz.im));
+        bad += !(same(pw.re, ww.re) && same(pw.im, ww.im));
+        cnt += 2;
+      }
+  printf("zpowi_pair %ld %ld\n", cnt, bad);
+
   // glibc sincos: the arc angles j dtheta + s dtheta (nsamples 1..64,
   // dyadic and general s), uniform angles over [-8, 8], every branch's
   // boundary neighbourhood and tiny arguments

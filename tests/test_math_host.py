@@ -49,7 +49,8 @@ def checker(tmp_path_factory):
     return res
 
 
-@pytest.mark.parametrize("name", ["cdiv", "cabs", "zpowi", "libm_sincos"])
+@pytest.mark.parametrize("name", ["cdiv", "cabs", "zpowi", "zpowi_pair",
+                                  "libm_sincos"])
 def test_bit_exact_against_host_routine(checker, name):
     checked, bad = checker[name]
     assert checked > 100000
