@@ -323,11 +323,15 @@ def serving_section(ctx, H, args, out):
     sweep = []
     if not args.no_sweep:
         # the plain path saturates at 1.0; the engine is swept beyond it
-        for f in (0.5, 1.0, 2.0, 4.0):
+        for f in (0.5, 1.0, 2.0, 4.0, 5.0, 6.0):
             rate = f * plain_rate
-            _, le, _, _, _ = eng.serve(stream, opts, arrivals(rate))
+            _, le, _, _, wall_f = eng.serve(stream, opts, arrivals(rate))
+            # the stream's paths over its span, first arrival to last
+            # result: the engine keeps up while this tracks the offered rate
             row = {"fraction_of_plain_saturation": f,
                    "requests_per_s": round(rate, 3),
+                   "offered_paths_per_s": round(f * paths / plain_s, 1),
+                   "served_paths_per_s": round(paths / wall_f, 1),
                    "engine_ms": percentiles(le)}
             sweep.append(row)
             log(f"sweep {f}: engine p50/p99 {row['engine_ms']['p50']}/"

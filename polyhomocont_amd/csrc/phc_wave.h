@@ -242,13 +242,18 @@ PHC_D void term_values(const Req& R, WP wt,
   }
 }
 
-// One derivative c e pw(e-1, j) pre(j-1) suf(j+1) (src/sparse.f90:273) of
-// a short-layout job record, FL factors per prefix and suffix.
-template <int FL, class WP>
-PHC_D cx job_product(const uint4 job, WP wt,
-                     const cx* __restrict__ T) {
+// The coefficient of a job record's term.
+template <class WP>
+PHC_D cx job_coef(const uint4 job, WP wt) {
   const auto& w = wt[job.x & 0xFFFFFu];
-  const cx c = mk(w.re, w.im);
+  return mk(w.re, w.im);
+}
+
+// One derivative c e pw(e-1, j) pre(j-1) suf(j+1) (src/sparse.f90:273) of
+// a short-layout job record with its term's coefficient c, FL factors per
+// prefix and suffix.
+template <int FL>
+PHC_D cx job_product(const uint4 job, const cx c, const cx* __restrict__ T) {
   const cx B = T[byte_of(job.y, 0)];
   cx Pf[FL], Sf[FL];
 #pragma unroll
@@ -294,7 +299,7 @@ PHC_D void jac_short(const Req& R, WP wt,
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const cx d = job_product<FL>(jc[u], wt, T);
+      const cx d = job_product<FL>(jc[u], job_coef(jc[u], wt), T);
       const int vb = vbase + 4 * (m + u);
       const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
       acc = cadd(acc, d);
@@ -336,7 +341,7 @@ PHC_D void jac_runs(const Req& R, WP wt,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint4 job = jc[u];
-      const cx d = job_product<FL>(job, wt, T);
+      const cx d = job_product<FL>(job, job_coef(job, wt), T);
       const int vb = (int)(job.w & 0xFFFFu);
       const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
       cur = cadd(cur, d);
@@ -356,14 +361,12 @@ PHC_D void jac_runs(const Req& R, WP wt,
 // products padded with pw(0, 0) = 1 to FLG factors each (the short layout
 // pads the same way); FLG = max(maxp, maxs) is a template argument, so the
 // factor loops carry no runtime bound.
-template <int FLG, class WP>
-PHC_D cx job_product_general(const uint4 job, WP wt,
+template <int FLG>
+PHC_D cx job_product_general(const uint4 job, const cx c,
                              const cx* __restrict__ T) {
   const int ex = (int)((job.x >> 20) & 255u);
   const uint64_t off = ((uint64_t)job.z << 32) | job.y;
   const int np = (int)(job.w & 15u), ns = (int)(job.w >> 4);
-  const auto& w = wt[job.x & 0xFFFFFu];
-  const cx c = mk(w.re, w.im);
   const cx B = T[woff(off, 0)];
   cx Pf[FLG], Sf[FLG];
 #pragma unroll
@@ -407,7 +410,7 @@ PHC_D void jac_general1(const Req& R, WP wt,
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const cx d = job_product_general<FLG>(jc[u], wt, T);
+      const cx d = job_product_general<FLG>(jc[u], job_coef(jc[u], wt), T);
       const int vb = vbase + 4 * (m + u);
       const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
       acc = cadd(acc, d);
@@ -922,8 +925,10 @@ PHC_D void unit_wave(const Req& R, WP wt,
     // patch is zero there, the table row holds any finite or non-finite
     // value -- and the sum keeps only the first N by selects, so that no
     // branch holds the loads back)
+    // (unrolled by four: a full unroll of the 16-lane grid's sum would hold
+    // all its loads in registers at once)
     cx psum = mk(0.0, 0.0);
-#pragma unroll
+#pragma unroll 4
     for (int jj = 0; jj < G; ++jj) {
       const cx pj = cmul(R.patch[jj], T[G + jj]);
       const cx sj = cadd(psum, pj);

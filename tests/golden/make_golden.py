@@ -5,7 +5,9 @@ this script is how they were made).
   python tests/golden/make_golden.py basic   katsura.json, endpoints.npz
   python tests/golden/make_golden.py c3      c3_full.npz
   python tests/golden/make_golden.py c4      c4_batch.npz
-  python tests/golden/make_golden.py all     all three
+  python tests/golden/make_golden.py c2      c2_full.npz
+  python tests/golden/make_golden.py serving serving_stream.npz
+  python tests/golden/make_golden.py all     all of them
 
 katsura.json   Katsura-3/-4 solution vectors held by the reference's own
                test (test/katsura.f90:22-137; computed there with
@@ -20,6 +22,16 @@ c3_full.npz    Every one of the 5040 paths of C3 and of its squared-factor
                (`endpoint`, src/solve.f90:217-252) and the result of the
                reference `solve` (src/solve.f90:154-215): counts,
                multiplicities and the per-path classification.
+c2_full.npz    Every one of the 2187 paths of C2 (dense n = 7, degree 3):
+               the reference's endpoint records and its `solve`'s counts
+               and per-path classification.
+serving_stream.npz
+               The 512 requests of the serving stream (seed 105, 239,777
+               paths): per request the reference `solve`'s counts and
+               multiplicities, per path its status, winding, end game
+               status, step counts and classification, and a SHA-256
+               digest of the result's doubles (final z, t, end game
+               error) per request.
 c4_batch.npz   All 512 seeded random sparse systems of the batched workload
                (743,290 paths): per path the discrete endpoint fields and
                the reference `solve`'s classification; per system the
@@ -119,6 +131,9 @@ def weighted_sum(seed, z):
 
 
 # ------------------------------------------------------------------ tasks
+_STREAMS = {}
+
+
 def _task(args):
     kind = args[0]
     import oracle
@@ -132,12 +147,31 @@ def _task(args):
         out = {f: getattr(e, f)[:count].copy() for f in
                ("status", "winding", "eg_status", "naccepted", "nrejected",
                 "t", "eg_error", "z")}
-    elif kind == "c3_solve":
+    elif kind == "c3_solve":  # also C2's solve
         _, name = args
         usys = workloads.config_system(name)
         st, res = R.solve(usys, opts)
         assert st == 0
         out = _solve_fields(res)
+    elif kind == "c2_endpoints":
+        _, first, count = args
+        e = R.endpoints(workloads.config_system("c2"), opts, first, count)
+        out = {f: getattr(e, f)[:count].copy() for f in
+               ("status", "winding", "eg_status", "naccepted", "nrejected",
+                "t", "eg_error", "z")}
+    elif kind == "serving":
+        _, count, i = args
+        if count not in _STREAMS:  # one stream per worker process
+            _STREAMS[count] = workloads.serving_stream(count)
+        usys = _STREAMS[count][i]
+        st, res = R.solve(usys, opts)
+        assert st == 0
+        np_ = res.summary.npaths
+        out = {f: res.p[f][:np_].copy() for f in
+               ("status", "winding", "eg_status", "naccepted", "nrejected")}
+        out["digest"] = endpoint_digest(res.p["z"][:np_], res.p["t"][:np_],
+                                        res.p["eg_error"][:np_])
+        out.update(_solve_fields(res))
     else:  # c4 system
         _, s = args
         usys = workloads.random_sparse(104 + s)
@@ -188,6 +222,62 @@ def run_tasks(tasks, cache, workers):
                   f"{_key(args)} {out['_seconds']:.1f} s", flush=True)
     return {_key(t): dict(np.load(os.path.join(cache, _key(t) + ".npz")))
             for t in tasks}
+
+
+C2_CHUNK = 81
+SERVING_REQUESTS = 512
+
+
+def c2_tasks():
+    return [("c3_solve", "c2")] + [("c2_endpoints", f, C2_CHUNK)
+                                   for f in range(0, 2187, C2_CHUNK)]
+
+
+def serving_tasks():
+    from polyhomocont_amd import workloads
+    stream = workloads.serving_stream(SERVING_REQUESTS)
+    order = sorted(range(len(stream)), key=lambda i: -stream[i].bezout)
+    return [("serving", SERVING_REQUESTS, i) for i in order]
+
+
+def write_c2(res):
+    parts = [res[_key(("c2_endpoints", f, C2_CHUNK))]
+             for f in range(0, 2187, C2_CHUNK)]
+    out = {}
+    for f in ("status", "winding", "eg_status"):
+        out[f] = np.concatenate([p[f] for p in parts]).astype(np.int8)
+    for f in ("naccepted", "nrejected"):
+        out[f] = np.concatenate([p[f] for p in parts]).astype(np.int32)
+    for f in ("t", "eg_error", "z"):
+        out[f] = np.concatenate([p[f] for p in parts])
+    sv = res[_key(("c3_solve", "c2"))]
+    out["counts"] = sv["counts"]
+    out["multiplicities"] = sv["multiplicities"]
+    for f in ("solution_index", "singular_index"):
+        out[f] = sv[f].astype(np.int32)
+    for f in ("at_infinity", "singular"):
+        out[f] = sv[f].astype(np.int8)
+    out["count_keys"] = np.array(COUNT_KEYS)
+    np.savez_compressed(os.path.join(HERE, "c2_full.npz"), **out)
+
+
+def write_serving(res):
+    R_ = SERVING_REQUESTS
+    per = [res[_key(("serving", R_, i))] for i in range(R_)]
+    out = {"count_keys": np.array(COUNT_KEYS)}
+    out["npaths"] = np.array([len(p["status"]) for p in per], np.int64)
+    out["offsets"] = np.concatenate([[0], np.cumsum(out["npaths"])])
+    for f in ("status", "winding", "eg_status", "at_infinity", "singular"):
+        out[f] = np.concatenate([p[f] for p in per]).astype(np.int8)
+    for f in ("naccepted", "nrejected", "solution_index", "singular_index"):
+        out[f] = np.concatenate([p[f] for p in per]).astype(np.int32)
+    out["counts"] = np.stack([p["counts"] for p in per])
+    mult = [p["multiplicities"] for p in per]
+    out["mult_offsets"] = np.concatenate([[0], np.cumsum([len(m) for m in mult])])
+    out["multiplicities"] = (np.concatenate(mult) if sum(len(m) for m in mult)
+                             else np.zeros(0, np.int32)).astype(np.int32)
+    out["digest"] = np.stack([p["digest"] for p in per])
+    np.savez_compressed(os.path.join(HERE, "serving_stream.npz"), **out)
 
 
 def c3_tasks():
@@ -249,23 +339,32 @@ def write_c4(res):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=("basic", "c3", "c4", "all"))
+    ap.add_argument("what", choices=("basic", "c3", "c4", "c2", "serving",
+                                     "all"))
     ap.add_argument("--cache", default="/tmp/phc_golden_cache")
     ap.add_argument("--workers", type=int, default=8)
     a = ap.parse_args()
     if a.what in ("basic", "all"):
         basic()
-    if a.what in ("c3", "c4", "all"):
+    if a.what != "basic":
         tasks = []
         if a.what in ("c3", "all"):
             tasks += c3_tasks()
         if a.what in ("c4", "all"):
             tasks += c4_tasks()
+        if a.what in ("c2", "all"):
+            tasks += c2_tasks()
+        if a.what in ("serving", "all"):
+            tasks += serving_tasks()
         res = run_tasks(tasks, a.cache, a.workers)
         if a.what in ("c3", "all"):
             write_c3(res)
         if a.what in ("c4", "all"):
             write_c4(res)
+        if a.what in ("c2", "all"):
+            write_c2(res)
+        if a.what in ("serving", "all"):
+            write_serving(res)
 
 
 if __name__ == "__main__":

@@ -75,3 +75,27 @@ def test_batch_classification_of_reference_records(host, ref):
         assert st == 0
         check(res, keys, F["counts"][s], F["multiplicities"][moff[s]:moff[s + 1]],
               {f: F[f][lo:hi] for f in CLASSIFIED}, hi - lo)
+
+
+@pytest.mark.parametrize("threads", [1, 8])
+def test_c2_classification_of_reference_records(host, threads):
+    """All 2187 C2 endpoint records of the reference, classified by the
+    host library, give the reference `solve`'s counts and per-path indices
+    (2187 finite regular solutions, every path its own solution)."""
+    F = load("c2_full.npz")
+    keys = [str(k) for k in F["count_keys"]]
+    usys = W.config_system("c2")
+    n = 2187
+    rec = abi.EndpointArrays(n, usys.n + 1)
+    for f in ("status", "winding", "eg_status", "naccepted", "nrejected",
+              "t", "eg_error", "z"):
+        getattr(rec, f)[:n] = F[f]
+    prev = host.set_threads(threads)
+    try:
+        st, res = host.classify(usys, abi.default_options(), rec)
+    finally:
+        host.set_threads(prev)
+    assert st == 0
+    check(res, keys, F["counts"], F["multiplicities"],
+          {f: F[f] for f in CLASSIFIED}, n)
+    assert res.summary.nsolutions == 2187

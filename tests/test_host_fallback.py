@@ -206,3 +206,41 @@ def test_dedup_window_equals_linear_scan(host):
     assert list(res.p["singular_index"][:npaths]) == want
     mult = np.bincount(want)[1:]
     assert list(res.multiplicities[:res.summary.nsingular]) == list(mult)
+
+
+def test_host_tracker_on_serving_requests_matches_fixture(host):
+    """The host tracker (set_device(-1)) on small requests of the serving
+    stream gives the reference `solve`'s results held by the fixture:
+    counts, every path's discrete fields and classification, and the digest
+    of the result's doubles."""
+    import os
+    from tests.golden.make_golden import (COUNT_KEYS, SERVING_REQUESTS,
+                                          endpoint_digest)
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                     "serving_stream.npz")
+    if not os.path.exists(p):
+        pytest.skip("serving_stream.npz not generated")
+    F = np.load(p)
+    stream = W.serving_stream(SERVING_REQUESTS)
+    off = F["offsets"]
+    small = [i for i in range(len(stream)) if stream[i].bezout <= 64][:6]
+    assert small
+    opts = abi.default_options()
+    host.set_device(-1)
+    try:
+        for i in small:
+            st, res = host.solve(stream[i], opts)
+            assert st == 0
+            lo, hi = off[i], off[i + 1]
+            n = hi - lo
+            c = res.counts()
+            assert [c[k] for k in COUNT_KEYS] == list(F["counts"][i]), i
+            for f in ("status", "winding", "eg_status", "naccepted",
+                      "nrejected", "solution_index", "singular_index",
+                      "at_infinity", "singular"):
+                assert np.array_equal(res.p[f][:n], F[f][lo:hi]), (i, f)
+            assert np.array_equal(
+                endpoint_digest(res.p["z"][:n], res.p["t"][:n],
+                                res.p["eg_error"][:n]), F["digest"][i]), i
+    finally:
+        host.set_device(0)
