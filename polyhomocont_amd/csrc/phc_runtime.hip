@@ -459,10 +459,7 @@ size_t split_lds_bytes(int G) {
   return (R * G * (G + 1) + (size_t)G * G + G) * sizeof(cx);
 }
 size_t grid_lds_bytes(int G, int expmax, int vmax) {
-  return state_lds_bytes(G) +
-         ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G +
-          2 * G) *
-             sizeof(cx);
+  return state_lds_bytes(G) + wave_lds_entries(G, expmax, vmax) * sizeof(cx);
 }
 
 // A prepared multi-request run for one lanes class, with device buffers.

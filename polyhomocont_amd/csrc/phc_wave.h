@@ -31,6 +31,7 @@
 // in the same order, so both units give identical bits.
 #pragma once
 
+#include <type_traits>
 #include <utility>
 
 namespace phc {
@@ -111,6 +112,29 @@ PHC_D int job_records(const Req& R) {
   return R.njobs * GridShape<G>::LP * (gf_runs(R.gfold) ? 1 : GridShape<G>::E);
 }
 
+//
+// The term values are dead once the job loop has read them, so with one
+// slot per lane (G = 4, 8) the arrays that the LU and the solves fill
+// afterwards, the factors A and the right-hand sides RB, live inside V when
+// V is large enough. A unit that used V so clears that span before it
+// returns, because V's padding must read as zero in the next unit's job
+// loop. (The 16-lane grid keeps them apart: its A holds the run sums
+// during the job loop, and its registers are at the two-wave limit.) Host
+// and device size the block with these same functions.
+PHC_HD constexpr int wave_slots(int G) {
+  return G / ((G * G < 64 ? G * G : 64) / G);
+}
+// Entries of V that A and RB take over, 0 when they are separate.
+PHC_HD constexpr int alias_entries(int G, int vmax) {
+  return wave_slots(G) == 1 && vmax >= 2 * G * G ? 2 * G * G : 0;
+}
+// Complex entries of a path's unit block: T, V, A and RB (unless inside V),
+// H and HT.
+PHC_HD constexpr size_t wave_lds_entries(int G, int expmax, int vmax) {
+  return (size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G -
+         (size_t)alias_entries(G, vmax) + 2 * G;
+}
+
 template <int G>
 struct WaveLds {
   cx* T;   // [(expmax + 1) G] power table, T[e G + j] = zq_j^e, T[j] = 1
@@ -120,17 +144,30 @@ struct WaveLds {
   cx* HT;  // [G] H_t,i
   cx* RB;  // [G G] right-hand side by slot (lane q, column e: q E + e)
   PHC_D static size_t bytes(int expmax, int vmax) {
-    return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G +
-            2 * G) *
-           sizeof(cx);
+    // the 16-lane grid never aliases; its size keeps the expression with
+    // which that kernel stays within two waves' registers
+    if constexpr (GridShape<G>::E > 1)
+      return ((size_t)(expmax + 1) * G + (size_t)vmax + (size_t)2 * G * G +
+              2 * G) *
+             sizeof(cx);
+    else
+      return wave_lds_entries(G, expmax, vmax) * sizeof(cx);
   }
+  // T | V | A | H | HT | RB, or T | V | H | HT with A and RB inside V
   PHC_D void bind(char* base, int expmax, int vmax) {
     T = (cx*)base;
     V = T + (size_t)(expmax + 1) * G;
-    A = V + vmax;
-    H = A + (size_t)G * G;
-    HT = H + G;
-    RB = HT + G;
+    if (alias_entries(G, vmax) > 0) {
+      A = V;
+      RB = V + G * G;
+      H = V + vmax;
+      HT = H + G;
+    } else {
+      A = V + vmax;
+      H = A + (size_t)G * G;
+      HT = H + G;
+      RB = HT + G;
+    }
   }
 };
 
@@ -821,7 +858,18 @@ PHC_D void unit_wave(const Req& R, WP wt,
   const int N = R.N, n = R.n;
   const int q = rep * G + lane;
   cx* __restrict__ T = W.T;
-  cx* __restrict__ V = W.V;
+  // restrict except where A and RB may live inside V (one slot per lane)
+  using VPtr = std::conditional_t<(E > 1), cx* __restrict__, cx*>;
+  VPtr V = W.V;
+  // the span of V that A and RB used, cleared before the unit returns (its
+  // padding must read as zero in the next unit's job loop)
+  auto release_v = [&]() {
+    if (E == 1 && W.A == W.V) {  // A and RB live in V's first 2 G^2 entries
+      wave_lds_fence();
+#pragma unroll
+      for (int i = q; i < 2 * G * G; i += LP) V[i] = mk(0.0, 0.0);
+    }
+  };
   const int fl_len = gf_fl(R.gfold);
   int col[E];
 #pragma unroll
@@ -918,15 +966,12 @@ PHC_D void unit_wave(const Req& R, WP wt,
     // (which that slot overwrites after reading)
     if (rep == 0 && lane < n)
       W.HT[lane] = csub(cmul(zdl, zl), cmul(cmul(bl, wdl), w));
-    // the patch row's sum(a*z) in index order, unrolled so that its loads
-    // and products do not wait on the additions; every lane forms it and
-    // lane n keeps it
-    // (every load and product is formed, entries past N included -- the
-    // patch is zero there, the table row holds any finite or non-finite
-    // value -- and the sum keeps only the first N by selects, so that no
-    // branch holds the loads back)
-    // (unrolled by four: a full unroll of the 16-lane grid's sum would hold
-    // all its loads in registers at once)
+    // The patch row's sum(a*z) in index order; every lane forms it and lane
+    // n keeps it. Every load and product is formed, entries past N
+    // included (the patch is zero there; the table row may hold anything),
+    // and the sum keeps the first N by selects, so no branch holds the
+    // loads back. Unrolled by four: a full unroll of the 16-lane grid's
+    // sum would hold all its loads in registers at once.
     cx psum = mk(0.0, 0.0);
 #pragma unroll 4
     for (int jj = 0; jj < G; ++jj) {
@@ -983,6 +1028,7 @@ PHC_D void unit_wave(const Req& R, WP wt,
     be = hn / fmax(rmax * zn, DBL_MIN);
     okout = 1;
     xout = mk(0.0, 0.0);
+    release_v();
     return;
   }
 
@@ -1018,6 +1064,7 @@ PHC_D void unit_wave(const Req& R, WP wt,
   okout = ok;
   if (ok != 1) {  // 0: zero pivot; 2: the fast LU's screen was inconclusive
     xout = mk(0.0, 0.0);
+    release_v();
     return;
   }
 
@@ -1049,6 +1096,7 @@ PHC_D void unit_wave(const Req& R, WP wt,
   }
   xout = solve_dispatch<G, (G == 4 ? 2 : G / 2 + 1)>(N, arow, bown, pdiag,
                                                       pos, row, lane, plb);
+  release_v();
 }
 
 }  // namespace phc
