@@ -234,8 +234,8 @@ uint4 short_record(const GridJob& jb) {
 // one step per four term values -- is a run of consecutive steps on one
 // lane; the runs are dealt longest first, each to the lane with the fewest
 // steps so far, so a lane holds one or more whole runs. Record w: bits
-// 0..15 the V index of the step's four term values (the zero row n vs for
-// derivative steps), bits 16..30 the run's sum index (i N + j, or n N + i
+// 0..15 the V index of the step's four term values (the four zeros at n vs
+// for derivative steps), bits 16..30 the run's sum index (i N + j, or n N + i
 // for F_i) and bit 31 on the run's last step, where the lane stores the
 // sum. Table [step][lane], the step count even (two steps per turn).
 bool build_runs(HostReq& h, int G, std::vector<WTerm>& wt,
@@ -458,6 +458,10 @@ size_t split_lds_bytes(int G) {
   const size_t R = 64 / G;
   return (R * G * (G + 1) + (size_t)G * G + G) * sizeof(cx);
 }
+// Entries of a path's term-value array V for a request's grid tables: its
+// n equations at vstride entries each, then the four zeros.
+int grid_v_entries(const Req& r) { return r.n * r.vstride + 4; }
+
 size_t grid_lds_bytes(int G, int expmax, int vmax) {
   return state_lds_bytes(G) + wave_lds_entries(G, expmax, vmax) * sizeof(cx);
 }
@@ -560,7 +564,7 @@ void plan_class(phc_ctx* ctx, ClassRun& cr) {
     if (grid) grid = build_grid(*h, G);
   if (grid)
     for (HostReq* h : cr.reqs)
-      cr.vmax = std::max(cr.vmax, (h->r.n + 1) * h->r.vstride);
+      cr.vmax = std::max(cr.vmax, grid_v_entries(h->r));
   bool can_split = G <= 8;
   for (HostReq* h : cr.reqs) can_split = can_split && h->r.N <= 64 / G;
   const int64_t row_waves = (std::max<int64_t>(cr.total, 1) * G + 63) / 64;

@@ -96,8 +96,9 @@ PHC_D int gf_fl(int g) { return (g >> 24) & 15; }
 PHC_D bool gf_runs(int g) { return (g >> 28) & 1; }
 
 // LDS of one path. V holds the term values of equation e at
-// V[e * vstride ...], zero-padded to vstride, then vstride zeros that the
-// Jacobian lanes add instead of values; it is cleared once per launch.
+// V[e * vstride ...], zero-padded to vstride, then four zeros that the
+// Jacobian lanes add instead of values (they read the same four every
+// step); it is cleared once per launch, and when a slot takes a new path.
 template <int G>
 struct GridShape {
   static constexpr int LP = (G * G < 64) ? G * G : 64;  // lanes per path
@@ -317,7 +318,8 @@ PHC_D cx job_product(const uint4 job, const cx c, const cx* __restrict__ T) {
 template <int G, int FL, class WP, class JP>
 PHC_D void jac_short(const Req& R, WP wt,
                      JP jobs, const cx* __restrict__ T,
-                     const cx* __restrict__ V, int q, int vbase, cx& acc) {
+                     const cx* __restrict__ V, int q, int vbase, int vstep,
+                     cx& acc) {
   static_assert(GridShape<G>::E == 1, "one slot per lane");
   constexpr int LP = GridShape<G>::LP, U = 2;
   const int M = R.njobs;
@@ -337,7 +339,7 @@ PHC_D void jac_short(const Req& R, WP wt,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const cx d = job_product<FL>(jc[u], job_coef(jc[u], wt), T);
-      const int vb = vbase + 4 * (m + u);
+      const int vb = vbase + vstep * (m + u);
       const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
       acc = cadd(acc, d);
       acc = cadd(acc, v0);
@@ -429,7 +431,7 @@ template <int G, int FLG, class WP, class JP>
 PHC_D void jac_general1(const Req& R, WP wt,
                         JP jobs,
                         const cx* __restrict__ T, const cx* __restrict__ V,
-                        int q, int vbase, cx& acc) {
+                        int q, int vbase, int vstep, cx& acc) {
   constexpr int LP = GridShape<G>::LP, U = 2;
   const int M = R.njobs;
   acc = mk(0.0, 0.0);
@@ -448,7 +450,7 @@ PHC_D void jac_general1(const Req& R, WP wt,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const cx d = job_product_general<FLG>(jc[u], job_coef(jc[u], wt), T);
-      const int vb = vbase + 4 * (m + u);
+      const int vb = vbase + vstep * (m + u);
       const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
       acc = cadd(acc, d);
       acc = cadd(acc, v0);
@@ -465,14 +467,15 @@ template <int G, int NC, class WP, class JP>
 PHC_D void jac_general1_dispatch(int flg, const Req& R, WP wt, JP jobs,
                                  const cx* __restrict__ T,
                                  const cx* __restrict__ V, int q, int vbase,
-                                 cx& acc) {
+                                 int vstep, cx& acc) {
   if constexpr (NC < 7) {
     if (flg > NC) {
-      jac_general1_dispatch<G, NC + 1>(flg, R, wt, jobs, T, V, q, vbase, acc);
+      jac_general1_dispatch<G, NC + 1>(flg, R, wt, jobs, T, V, q, vbase,
+                                       vstep, acc);
       return;
     }
   }
-  jac_general1<G, NC>(R, wt, jobs, T, V, q, vbase, acc);
+  jac_general1<G, NC>(R, wt, jobs, T, V, q, vbase, vstep, acc);
 }
 
 // The same with the general job layout (any number of factors up to 8).
@@ -929,17 +932,20 @@ PHC_D void unit_wave(const Req& R, WP wt,
       jac_general<G>(R, wt, jobs, T, V, q, vbase, vcount, acc);
     }
   } else {
+    // F lanes walk their equation's values four per step; the others add
+    // the four zeros after the last equation, every step
+    const int vstep = fl[0] ? 4 : 0;
     if (fl_len == 1)
-      jac_short<G, 1>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
+      jac_short<G, 1>(R, wt, jobs, T, V, q, vbase[0], vstep, acc[0]);
     else if (fl_len == 2)
-      jac_short<G, 2>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
+      jac_short<G, 2>(R, wt, jobs, T, V, q, vbase[0], vstep, acc[0]);
     else if (fl_len == 3)
-      jac_short<G, 3>(R, wt, jobs, T, V, q, vbase[0], acc[0]);
+      jac_short<G, 3>(R, wt, jobs, T, V, q, vbase[0], vstep, acc[0]);
     else
       jac_general1_dispatch<G, 1>(
           gf_maxp(R.gfold) > gf_maxs(R.gfold) ? gf_maxp(R.gfold)
                                               : gf_maxs(R.gfold),
-          R, wt, jobs, T, V, q, vbase[0], acc[0]);
+          R, wt, jobs, T, V, q, vbase[0], vstep, acc[0]);
   }
 
   // 4. homotopy (src/homotopy.f90:491-511) and the patch row; the slot
