@@ -69,9 +69,11 @@ PHC_HD cx cmul(cx a, cx b) {
 PHC_HD cx cscale(double s, cx a) { return mk(s * a.re, s * a.im); }
 
 // C99 Annex G recovery of infinities and zeros for __divdc3 when the
-// formula gave NaN + i NaN (c, d already scaled by 2^-ilogbw).
-PHC_HD_COLD cx cdiv_special(double a, double b, double c, double d,
-                            double denom, double logbw, double x, double y) {
+// formula gave NaN + i NaN (c, d already scaled by 2^-ilogbw). The divisor
+// was infinite (logb = +inf) exactly when c or d is: a finite divisor's
+// c and d are scaled to finite values, an infinite one is not scaled.
+PHC_HD cx cdiv_special(double a, double b, double c, double d,
+                       double denom, double x, double y) {
   {
     if (denom == 0.0 && (!isnan(a) || !isnan(b))) {
       x = copysign(INFINITY, c) * a;
@@ -81,7 +83,7 @@ PHC_HD_COLD cx cdiv_special(double a, double b, double c, double d,
       b = copysign(isinf(b) ? 1.0 : 0.0, b);
       x = INFINITY * (a * c + b * d);
       y = INFINITY * (b * c - a * d);
-    } else if (isinf(logbw) && logbw > 0.0 && isfinite(a) && isfinite(b)) {
+    } else if ((isinf(c) || isinf(d)) && isfinite(a) && isfinite(b)) {
       c = copysign(isinf(c) ? 1.0 : 0.0, c);
       d = copysign(isinf(d) ? 1.0 : 0.0, d);
       x = 0.0 * (a * c + b * d);
@@ -95,31 +97,117 @@ PHC_HD_COLD cx cdiv_special(double a, double b, double c, double d,
 // scaled c and d, c^2 + d^2), which depends only on the divisor, and the
 // quotient for a numerator. cdiv(a, b) == cdiv_apply(a, cdiv_prep(b)) bit
 // for bit; a pivot used for several quotients is prepared once.
+//
+// logb(m), m = max(|c|, |d|), is finite exactly when m is finite and
+// nonzero; it is then the exponent of m (subnormals as if normalized),
+// which is what frexp's exponent minus one gives (the device's own logb is
+// formed that way). Otherwise the divisor is not scaled.
+//
+// On the device the divisor also carries `rden`, the reciprocal of
+// c^2 + d^2 as the compiler's IEEE division sequence refines it (v_rcp_f64
+// and two Newton steps), so that a quotient only runs the sequence's last
+// three operations (see qdiv_fast). A divisor that is not scaled has
+// c^2 + d^2 = 0, inf or NaN, whose refined reciprocal is NaN: its
+// quotients take the full division.
 struct cdiv_den {
-  double c, d, denom, logbw;
+  double c, d, denom, rden;
   int ilogbw;
 };
-PHC_HD cdiv_den cdiv_prep(cx den) {
+PHC_HD int exponent_of(double m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_frexp_exp(m) - 1;
+#else
+  return ilogb(m);
+#endif
+}
+// The scaling alone (rden not formed).
+PHC_HD cdiv_den cdiv_scale(cx den) {
   cdiv_den p;
-  p.c = den.re;
-  p.d = den.im;
-  p.ilogbw = 0;
-  p.logbw = logb(fmax(fabs(p.c), fabs(p.d)));
-  if (isfinite(p.logbw)) {
-    p.ilogbw = (int)p.logbw;
-    p.c = ldexp(p.c, -p.ilogbw);
-    p.d = ldexp(p.d, -p.ilogbw);
-  }
+  const double m = fmax(fabs(den.re), fabs(den.im));
+  const bool scale = isfinite(m) && m != 0.0;
+  p.ilogbw = scale ? exponent_of(m) : 0;
+  // ldexp by 0 returns its argument, zeros, infinities and NaNs included
+  p.c = ldexp(den.re, -p.ilogbw);
+  p.d = ldexp(den.im, -p.ilogbw);
   p.denom = p.c * p.c + p.d * p.d;
+  p.rden = 0.0;
   return p;
 }
+// rden of a scaled divisor (a pivot searched by all lanes is scaled by
+// every candidate and gets its reciprocal once, after the broadcast).
+PHC_HD void cdiv_recip(cdiv_den& p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // LLVM's f64 division: r = rcp(y); e = fma(-y, r, 1); r = fma(r, e, r);
+  // e = fma(-y, r, 1); r = fma(r, e, r)
+  double r = __builtin_amdgcn_rcp(p.denom);
+  double e = fma(-p.denom, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-p.denom, r, 1.0);
+  p.rden = fma(r, e, r);
+#else
+  (void)p;
+#endif
+}
+PHC_HD cdiv_den cdiv_prep(cx den) {
+  cdiv_den p = cdiv_scale(den);
+  cdiv_recip(p);
+  return p;
+}
+
+// n / denom on the device, bit for bit what the compiler's division gives
+// where its scaling steps do nothing. The compiler divides x by y as
+//   s = div_scale(y); r = the refined reciprocal of s (cdiv_prep);
+//   u = div_scale(x); q = u r; e = fma(-s, q, u);
+//   q = div_fmas(e, r, q); result = div_fixup(q, y, x).
+// For y = denom in [1, 8) (a scaled divisor) and a numerator with
+// 2^-969 <= |x| < 2^768, div_scale returns its operand and clears VCC, so
+// div_fmas is a plain fma, and div_fixup returns q for a normal quotient:
+// the result is fma(e, r, x r) with e = fma(-y, x r, x). This forms the
+// same with the residual's sign flipped, -e = fma(y, x r, -x) (exact), so
+// that a zero numerator keeps its sign as div_fixup would give it
+// (+0 / y = +0, -0 / y = -0). qdiv_fast_ok tells where it applies.
+PHC_HD double qdiv_fast(double n, const cdiv_den& p) {
+  const double q = n * p.rden;
+  const double me = fma(p.denom, q, -n);
+  return fma(-me, p.rden, q);
+}
+PHC_HD bool qdiv_fast_ok(double n, double q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // frexp exponent in [-968, 768], or 0 for a zero; infinite or NaN
+  // numerators (exponent 0 too) and unscaled divisors give a NaN q
+  const int e = __builtin_amdgcn_frexp_exp(n);
+  return (unsigned int)(e + 968) <= 1736u && !isnan(q);
+#else
+  (void)n;
+  (void)q;
+  return false;
+#endif
+}
+
+// The full __divdc3 quotient: IEEE divisions, then Annex G. Inline: on
+// the device it sits in a branch that the lanes skip when every quotient
+// took qdiv_fast (an out-of-line call here would make the caller keep its
+// registers clear of the callee's, which pushes the 16-lane grid kernels
+// past two waves per SIMD).
+PHC_HD cx cdiv_full(double a, double b, double nr, double ni, double c,
+                    double d, double denom, int ilogbw) {
+  const double x = ldexp(nr / denom, -ilogbw);
+  const double y = ldexp(ni / denom, -ilogbw);
+  if (isnan(x) && isnan(y)) return cdiv_special(a, b, c, d, denom, x, y);
+  return mk(x, y);
+}
+
 PHC_HD cx cdiv_apply(cx num, const cdiv_den& p) {
   const double a = num.re, b = num.im;
-  const double x = ldexp((a * p.c + b * p.d) / p.denom, -p.ilogbw);
-  const double y = ldexp((b * p.c - a * p.d) / p.denom, -p.ilogbw);
-  if (isnan(x) && isnan(y))
-    return cdiv_special(a, b, p.c, p.d, p.denom, p.logbw, x, y);
-  return mk(x, y);
+  const double nr = a * p.c + b * p.d;
+  const double ni = b * p.c - a * p.d;
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double qr = qdiv_fast(nr, p), qi = qdiv_fast(ni, p);
+  // both tests evaluated (no short circuit), one branch
+  if (qdiv_fast_ok(nr, qr) & qdiv_fast_ok(ni, qi))
+    return mk(ldexp(qr, -p.ilogbw), ldexp(qi, -p.ilogbw));
+#endif
+  return cdiv_full(a, b, nr, ni, p.c, p.d, p.denom, p.ilogbw);
 }
 PHC_HD cx cdiv(cx num, cx den) { return cdiv_apply(num, cdiv_prep(den)); }
 

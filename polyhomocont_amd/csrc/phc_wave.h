@@ -199,15 +199,16 @@ template <int LP>
 PHC_D cx pshfl_c(cx v, int src) {
   return mk(__shfl(v.re, src, LP), __shfl(v.im, src, LP));
 }
-// A divisor prepared by lane `src` (cdiv_prep of its own entry).
+// A divisor scaled by lane `src` (cdiv_scale of its own entry), its
+// reciprocal formed once after the broadcast.
 template <int LP>
 PHC_D cdiv_den pbcast_den(const cdiv_den& v, int src) {
   cdiv_den r;
   r.c = pbcast_d<LP>(v.c, src);
   r.d = pbcast_d<LP>(v.d, src);
   r.denom = pbcast_d<LP>(v.denom, src);
-  r.logbw = pbcast_d<LP>(v.logbw, src);
   r.ilogbw = pbcast_i<LP>(v.ilogbw, src);
+  cdiv_recip(r);
   return r;
 }
 
@@ -593,8 +594,8 @@ PHC_D int lu_slots(int N, cx (&a)[GridShape<G>::E],
         if (!ok || k >= N) break;
         const cx aik = pshfl_c<LP>(a[ke], kr * G + lane);
         // every lane prepares its own entry as a divisor while the pivot is
-        // searched; the winner's is broadcast (cdiv_prep of the pivot)
-        const cdiv_den mine = cdiv_prep(a[ke]);
+        // searched; the winner's is broadcast (the pivot's scaling)
+        const cdiv_den mine = cdiv_scale(a[ke]);
         double bv;
         int bp, bl;
         pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, lane,
@@ -644,7 +645,7 @@ PHC_D int lu_one_slot_exact(int N, cx& a, const bool in, const int col,
     // column k of this lane's row, fetched while the pivot is searched:
     // every group forms its row's multiplier itself
     const cx aik = pshfl_c<LP>(a, k * G + lane);
-    const cdiv_den mine = cdiv_prep(a);  // the winner's is the pivot's
+    const cdiv_den mine = cdiv_scale(a);  // the winner's is the pivot's
     double bv;
     int bp, bl;
     pivot_search<G, LP>(in && rep == k && pos >= k, a, pos, lane, k * G, bv,
@@ -697,7 +698,7 @@ PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
     const cx aik = T[k * G + lane];
     const int gbase = k * G;
     const bool cand = in && rep == k && pos >= k;
-    const cdiv_den mine = cdiv_prep(a);  // the winner's is the pivot's
+    const cdiv_den mine = cdiv_scale(a);  // the winner's is the pivot's
     const double sq = a.re * a.re + a.im * a.im;
     const double sv = pbcast_d<LP>(gmax<G>(cand ? sq : -1.0), gbase);
     const unsigned long long hit = (__ballot(cand && sq == sv) >> shift) & pmask;
@@ -753,7 +754,7 @@ PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
         const cx aik = T[k * G + lane];
         const int gbase = kr * G;
         const bool cand = in[ke] && rep == kr && pos >= k;
-        const cdiv_den mine = cdiv_prep(a[ke]);  // the winner's: the pivot's
+        const cdiv_den mine = cdiv_scale(a[ke]);  // the winner's: the pivot's
         const double sq = a[ke].re * a[ke].re + a[ke].im * a[ke].im;
         const double sv = pbcast_d<LP>(gmax<G>(cand ? sq : -1.0), gbase);
         const unsigned long long hit = __ballot(cand && sq == sv);
@@ -810,22 +811,26 @@ PHC_D cx solve_rows(const cx (&arow)[G], const cx bown, const cdiv_den pdiag,
     }
   });
   const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
-  cx xs[NC];
+  // back substitution: the row at position i needs
+  // sum_{j > i} a(i, j) x_j in ascending j; each product a(i, j) x_j is
+  // formed once, when x_j is known (px[j]), and step i adds them in order
+  cx px[NC];
   cx xmine = mk(0.0, 0.0);
   static_for<NC>([&](auto ic) {
     constexpr int i = NC - 1 - decltype(ic)::value;
     cx num = yfin;
     if constexpr (i + 1 < NC) {
-      cx sm = cmul(arow[i + 1], xs[i + 1]);
+      cx sm = px[i + 1];
       static_for<NC>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        if constexpr (j >= i + 2) sm = cadd(sm, cmul(arow[j], xs[j]));
+        if constexpr (j >= i + 2) sm = cadd(sm, px[j]);
       });
       num = csub(yfin, sm);
     }
     const cx cand = cdiv_apply(num, pdiag);
-    xs[i] = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
-    if (lane == i) xmine = xs[i];
+    const cx xi = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
+    if (lane == i) xmine = xi;
+    if constexpr (i > 0) px[i] = cmul(arow[i], xi);
   });
   return xmine;
 }
