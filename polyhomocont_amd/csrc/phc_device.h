@@ -268,13 +268,16 @@ PHC_D void wave_lds_fence() {
 // seg_t (src/tracker.f90:142-162; seg_dt, :164-177, is formed where the
 // tangent needs it). Lines are real (all lines the
 // tracker uses are); arcs evaluate cos and sin of theta0 + s dtheta as
-// glibc's sincos does.
+// glibc's sincos does. UNIFORM: every lane of the wavefront tracks this
+// path (one path per wavefront), so the angle is taken as a scalar.
+template <bool UNIFORM = false>
 PHC_D cx seg_t(const PathScalars& S, double s) {
   if (S.sk == 0) {
     if (s >= 1.0) return mk(S.sb, 0.0);
     return mk(S.sa + s * (S.sb - S.sa), 0.0);
   }
-  const cx cs = libm_sincos(S.sa + s * S.sdth);
+  const double th = S.sa + s * S.sdth;
+  const cx cs = UNIFORM ? libm_sincos_uniform(th) : libm_sincos(th);
   return cscale(S.sb, cs);  // radius in sb
 }
 
@@ -1243,7 +1246,7 @@ __global__ void __launch_bounds__(WG)
         dt = mk(0.0, 0.0);
         if (S.pc != PC_BE) {
           const double sv = (S.pc == PC_TAN) ? S.se : S.sn;
-          t = seg_t(S, sv);
+          t = seg_t<LP == 64>(S, sv);
           if (S.pc == PC_TAN)
             dt = (S.sk == 0) ? mk(S.sb - S.sa, 0.0)
                              : mk(0.0 * t.re - S.sdth * t.im,

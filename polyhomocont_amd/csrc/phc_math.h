@@ -72,8 +72,8 @@ PHC_HD cx cscale(double s, cx a) { return mk(s * a.re, s * a.im); }
 // formula gave NaN + i NaN (c, d already scaled by 2^-ilogbw). The divisor
 // was infinite (logb = +inf) exactly when c or d is: a finite divisor's
 // c and d are scaled to finite values, an infinite one is not scaled.
-PHC_HD cx cdiv_special(double a, double b, double c, double d,
-                       double denom, double x, double y) {
+PHC_HD_COLD cx cdiv_special(double a, double b, double c, double d,
+                            double denom, double x, double y) {
   {
     if (denom == 0.0 && (!isnan(a) || !isnan(b))) {
       x = copysign(INFINITY, c) * a;
@@ -95,8 +95,8 @@ PHC_HD cx cdiv_special(double a, double b, double c, double d,
 
 // compiler-rt __divdc3, in two parts: the divisor's scaling (logb, the
 // scaled c and d, c^2 + d^2), which depends only on the divisor, and the
-// quotient for a numerator. cdiv(a, b) == cdiv_apply(a, cdiv_prep(b)) bit
-// for bit; a pivot used for several quotients is prepared once.
+// quotient for a numerator. cdiv(a, b) == cdiv_apply(a, cdiv_scale(b)) bit
+// for bit; a pivot used for several quotients is scaled once.
 //
 // logb(m), m = max(|c|, |d|), is finite exactly when m is finite and
 // nonzero; it is then the exponent of m (subnormals as if normalized),
@@ -106,9 +106,9 @@ PHC_HD cx cdiv_special(double a, double b, double c, double d,
 // On the device the divisor also carries `rden`, the reciprocal of
 // c^2 + d^2 as the compiler's IEEE division sequence refines it (v_rcp_f64
 // and two Newton steps), so that a quotient only runs the sequence's last
-// three operations (see qdiv_fast). A divisor that is not scaled has
-// c^2 + d^2 = 0, inf or NaN, whose refined reciprocal is NaN: its
-// quotients take the full division.
+// three operations (see qdiv_fast, cdiv_apply_fast). A divisor that is not
+// scaled has c^2 + d^2 = 0, inf or NaN, whose refined reciprocal is NaN:
+// its quotients fail qdiv_fast_ok and take the full division.
 struct cdiv_den {
   double c, d, denom, rden;
   int ilogbw;
@@ -148,6 +148,7 @@ PHC_HD void cdiv_recip(cdiv_den& p) {
   (void)p;
 #endif
 }
+// Scaling and reciprocal, for cdiv_apply_fast.
 PHC_HD cdiv_den cdiv_prep(cx den) {
   cdiv_den p = cdiv_scale(den);
   cdiv_recip(p);
@@ -156,7 +157,7 @@ PHC_HD cdiv_den cdiv_prep(cx den) {
 
 // n / denom on the device, bit for bit what the compiler's division gives
 // where its scaling steps do nothing. The compiler divides x by y as
-//   s = div_scale(y); r = the refined reciprocal of s (cdiv_prep);
+//   s = div_scale(y); r = the refined reciprocal of s (cdiv_recip);
 //   u = div_scale(x); q = u r; e = fma(-s, q, u);
 //   q = div_fmas(e, r, q); result = div_fixup(q, y, x).
 // For y = denom in [1, 8) (a scaled divisor) and a numerator with
@@ -184,32 +185,35 @@ PHC_HD bool qdiv_fast_ok(double n, double q) {
 #endif
 }
 
-// The full __divdc3 quotient: IEEE divisions, then Annex G. Inline: on
-// the device it sits in a branch that the lanes skip when every quotient
-// took qdiv_fast (an out-of-line call here would make the caller keep its
-// registers clear of the callee's, which pushes the 16-lane grid kernels
-// past two waves per SIMD).
-PHC_HD cx cdiv_full(double a, double b, double nr, double ni, double c,
-                    double d, double denom, int ilogbw) {
-  const double x = ldexp(nr / denom, -ilogbw);
-  const double y = ldexp(ni / denom, -ilogbw);
-  if (isnan(x) && isnan(y)) return cdiv_special(a, b, c, d, denom, x, y);
+// The __divdc3 quotient: IEEE divisions, then Annex G.
+PHC_HD cx cdiv_apply(cx num, const cdiv_den& p) {
+  const double a = num.re, b = num.im;
+  const double x = ldexp((a * p.c + b * p.d) / p.denom, -p.ilogbw);
+  const double y = ldexp((b * p.c - a * p.d) / p.denom, -p.ilogbw);
+  if (isnan(x) && isnan(y))
+    return cdiv_special(a, b, p.c, p.d, p.denom, x, y);
   return mk(x, y);
 }
+PHC_HD cx cdiv(cx num, cx den) { return cdiv_apply(num, cdiv_scale(den)); }
 
-PHC_HD cx cdiv_apply(cx num, const cdiv_den& p) {
+// The same quotient through qdiv_fast (the divisor's rden formed), for
+// the linearization unit's LU and solves: it clears `ok` where qdiv_fast
+// does not apply to either part, and the unit is then redone with
+// cdiv_apply. Keeping the full division out of this path keeps it out of
+// the unit's hot code. (Host: cdiv_apply.)
+PHC_HD cx cdiv_apply_fast(cx num, const cdiv_den& p, bool& ok) {
+#if defined(__HIP_DEVICE_COMPILE__)
   const double a = num.re, b = num.im;
   const double nr = a * p.c + b * p.d;
   const double ni = b * p.c - a * p.d;
-#if defined(__HIP_DEVICE_COMPILE__)
   const double qr = qdiv_fast(nr, p), qi = qdiv_fast(ni, p);
-  // both tests evaluated (no short circuit), one branch
-  if (qdiv_fast_ok(nr, qr) & qdiv_fast_ok(ni, qi))
-    return mk(ldexp(qr, -p.ilogbw), ldexp(qi, -p.ilogbw));
+  ok = ok & qdiv_fast_ok(nr, qr) & qdiv_fast_ok(ni, qi);
+  return mk(ldexp(qr, -p.ilogbw), ldexp(qi, -p.ilogbw));
+#else
+  (void)ok;
+  return cdiv_apply(num, p);
 #endif
-  return cdiv_full(a, b, nr, ni, p.c, p.d, p.denom, p.ilogbw);
 }
-PHC_HD cx cdiv(cx num, cx den) { return cdiv_apply(num, cdiv_prep(den)); }
 
 // Correction step of Borges' algorithm for 0 < ay <= ax, no overflow.
 // Both cases of the correction are formed and one is selected, so lanes
@@ -376,7 +380,7 @@ PHC_HD double sin_part(double x, double dx) {
 }  // namespace sincos_detail
 
 // (cos x, sin x) for finite |x| < 105414350 (arc angles lie in [0, 2 pi]).
-PHC_HD_COLD cx libm_sincos(double x) {
+PHC_HD cx sincos_body(double x) {
   using namespace sincos_detail;
   const uint32_t k = (uint32_t)(bits_of(x) >> 32) & 0x7fffffffu;
   if (k < 0x3e400000u) return mk(1.0, x);  // |x| < 2^-27
@@ -414,6 +418,25 @@ PHC_HD_COLD cx libm_sincos(double x) {
     case 2: return mk(-cv, sv);
     default: return mk(sv, -cv);
   }
+}
+
+// Out of line: lanes may hold different angles.
+PHC_HD_COLD cx libm_sincos(double x) { return sincos_body(x); }
+
+// The same for an angle that every lane of the wavefront holds (one path
+// per wavefront): taken into scalar registers first, so the branches are
+// scalar and the table rows come through the scalar cache. Out of line
+// too: inlined, it costs the 16-lane grid kernels their second wave per
+// SIMD.
+PHC_HD_COLD cx libm_sincos_uniform(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint64_t b = sincos_detail::bits_of(x);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  const uint64_t u = ((uint64_t)hi << 32) | lo;
+  __builtin_memcpy(&x, &u, sizeof x);
+#endif
+  return sincos_body(x);
 }
 
 }  // namespace phc

@@ -199,8 +199,8 @@ template <int LP>
 PHC_D cx pshfl_c(cx v, int src) {
   return mk(__shfl(v.re, src, LP), __shfl(v.im, src, LP));
 }
-// A divisor scaled by lane `src` (cdiv_scale of its own entry), its
-// reciprocal formed once after the broadcast.
+// A divisor scaled by lane `src` (cdiv_scale of its own entry); the fast
+// LU forms its reciprocal once, after the broadcast.
 template <int LP>
 PHC_D cdiv_den pbcast_den(const cdiv_den& v, int src) {
   cdiv_den r;
@@ -208,7 +208,7 @@ PHC_D cdiv_den pbcast_den(const cdiv_den& v, int src) {
   r.d = pbcast_d<LP>(v.d, src);
   r.denom = pbcast_d<LP>(v.denom, src);
   r.ilogbw = pbcast_i<LP>(v.ilogbw, src);
-  cdiv_recip(r);
+  r.rden = 0.0;
   return r;
 }
 
@@ -689,6 +689,7 @@ PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
   const int shift = LP == 64 ? 0 : (int)(__lane_id() / LP) * LP;
   const unsigned long long pmask = LP == 64 ? ~0ull : ((1ull << LP) - 1ull);
   bool sure = true;
+  bool dok = true;  // every quotient within cdiv_apply_fast's range
 #pragma unroll
   for (int k = 0; k < G; ++k) {
     if (k >= N) break;
@@ -712,17 +713,20 @@ PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
     const int bl = wl - gbase;
     const int bp = pbcast_i<LP>(pos, wl);
     plb |= (uint64_t)(bl & 15) << (4 * k);
-    const cdiv_den pd = pbcast_den<LP>(mine, wl);
+    cdiv_den pd = pbcast_den<LP>(mine, wl);
+    cdiv_recip(pd);
     const cx pr = T[rep * G + bl];
     int np = pos;
     if (pos == k) np = bp;
     if (lane == bl) np = k;
     pos = np;
-    const cx li = cdiv_apply(aik, pd);
+    const cx li = cdiv_apply_fast(aik, pd, dok);
     if (in && rep == k && pos > k) a = li;
     const cx upd = csub(a, cmul(li, pr));
     if (in && col > k && pos > k) a = upd;
   }
+  // a quotient outside qdiv_fast's range: redone exactly too
+  sure = sure && ((__ballot(!dok) >> shift) & pmask) == 0;
   return sure ? 1 : 2;
 }
 
@@ -740,6 +744,7 @@ PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
                 E = GridShape<G>::E;
   static_assert(LP == 64, "one path per wavefront");
   bool sure = true;
+  bool dok = true;  // every quotient within cdiv_apply_fast's range
   static_for<E>([&](auto ec) {
     constexpr int ke = decltype(ec)::value;
     if constexpr (ke < EU) {
@@ -765,12 +770,13 @@ PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
         const int bl = wl - gbase;
         const int bp = pbcast_i<LP>(pos, wl);
         plb |= (uint64_t)(bl & 15) << (4 * k);
-        const cdiv_den pd = pbcast_den<LP>(mine, wl);
+        cdiv_den pd = pbcast_den<LP>(mine, wl);
+        cdiv_recip(pd);
         int np = pos;
         if (pos == k) np = bp;
         if (lane == bl) np = k;
         pos = np;
-        const cx li = cdiv_apply(aik, pd);
+        const cx li = cdiv_apply_fast(aik, pd, dok);
         if (in[ke] && rep == kr && pos > k) a[ke] = li;
 #pragma unroll
         for (int e = ke; e < EU; ++e) {
@@ -780,6 +786,8 @@ PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
       }
     }
   });
+  // a quotient outside qdiv_fast's range: redone exactly too
+  sure = sure && __ballot(!dok) == 0;
   return sure ? 1 : 2;
 }
 
@@ -790,9 +798,13 @@ PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
 // PL packed in plb). Sums in the reference's order, as in the
 // row-per-lane unit. With NC a constant the steps carry no row-count
 // tests (such tests inside the unrolled steps cost as much as the steps).
-template <int G, int NC>
+//
+// FAST: the quotients through cdiv_apply_fast (pdiag carries its rden);
+// `dok` is cleared when the quotient of this lane's own row (the one its
+// step broadcasts) fell outside that range, and the unit is then redone.
+template <int G, int NC, bool FAST>
 PHC_D cx solve_rows(const cx (&arow)[G], const cx bown, const cdiv_den pdiag,
-                    int pos, bool row, int lane, uint64_t plb) {
+                    int pos, bool row, int lane, uint64_t plb, bool& dok) {
   constexpr int LP = GridShape<G>::LP;
   auto rowcast = [&](cx v, int r) -> cx {
     if (LP == 64) return pbcast_c<LP>(v, r);  // group 0, lane r
@@ -827,7 +839,14 @@ PHC_D cx solve_rows(const cx (&arow)[G], const cx bown, const cdiv_den pdiag,
       });
       num = csub(yfin, sm);
     }
-    const cx cand = cdiv_apply(num, pdiag);
+    cx cand;
+    if constexpr (FAST) {
+      bool qok = true;
+      cand = cdiv_apply_fast(num, pdiag, qok);
+      dok = dok & (qok | (pos != i));
+    } else {
+      cand = cdiv_apply(num, pdiag);
+    }
     const cx xi = rowcast(cand, (int)((plb >> (4 * i)) & 15u));
     if (lane == i) xmine = xi;
     if constexpr (i > 0) px[i] = cmul(arow[i], xi);
@@ -837,16 +856,16 @@ PHC_D cx solve_rows(const cx (&arow)[G], const cx bown, const cdiv_den pdiag,
 
 // solve_rows for the path's N among NC .. G (a lanes class of G serves
 // N from G / 2 + 1 to G; the 4-lane class from 2).
-template <int G, int NC>
+template <int G, int NC, bool FAST>
 PHC_D cx solve_dispatch(int N, const cx (&arow)[G], const cx bown,
                         const cdiv_den pdiag, int pos, bool row, int lane,
-                        uint64_t plb) {
+                        uint64_t plb, bool& dok) {
   if constexpr (NC < G) {
     if (N != NC)
-      return solve_dispatch<G, NC + 1>(N, arow, bown, pdiag, pos, row, lane,
-                                       plb);
+      return solve_dispatch<G, NC + 1, FAST>(N, arow, bown, pdiag, pos, row,
+                                             lane, plb, dok);
   }
-  return solve_rows<G, NC>(arow, bown, pdiag, pos, row, lane, plb);
+  return solve_rows<G, NC, FAST>(arow, bown, pdiag, pos, row, lane, plb, dok);
 }
 
 // Factors that are exactly one: the reference multiplies by pw(0, j) = 1
@@ -1095,9 +1114,10 @@ PHC_D void unit_wave(const Req& R, WP wt,
     arow[j] = row ? W.A[lane * G + j] : mk(1.0, 0.0);
   // The quotient of back-substitution step i is taken from the lane whose
   // row is at logical position i, where arow[i] is that row's diagonal
-  // entry: its divisor scaling is prepared once, off the step chain.
-  const cdiv_den pdiag =
-      cdiv_prep(row ? W.A[lane * G + (pos < G ? pos : 0)] : mk(1.0, 0.0));
+  // entry: its divisor scaling (and, for the fast quotients, its
+  // reciprocal) is prepared once, off the step chain.
+  const cx diag = row ? W.A[lane * G + (pos < G ? pos : 0)] : mk(1.0, 0.0);
+  const cdiv_den pdiag = EXACT ? cdiv_scale(diag) : cdiv_prep(diag);
   cx bown = mk(0.0, 0.0);
   if (row) {
     // the slot that holds row `lane`'s right-hand side
@@ -1105,8 +1125,16 @@ PHC_D void unit_wave(const Req& R, WP wt,
     const int orow = lane < n ? G - 1 : n;          // its row
     bown = W.RB[((oc % RG) * G + orow) * E + oc / RG];
   }
-  xout = solve_dispatch<G, (G == 4 ? 2 : G / 2 + 1)>(N, arow, bown, pdiag,
-                                                      pos, row, lane, plb);
+  bool dok = true;
+  xout = solve_dispatch<G, (G == 4 ? 2 : G / 2 + 1), !EXACT>(
+      N, arow, bown, pdiag, pos, row, lane, plb, dok);
+  if constexpr (!EXACT) {
+    // a quotient outside cdiv_apply_fast's range: the caller redoes the
+    // unit with the exact LU and solves (per path, as for the LU's screen)
+    const int shift = LP == 64 ? 0 : (int)(__lane_id() / LP) * LP;
+    const unsigned long long pmask = LP == 64 ? ~0ull : ((1ull << LP) - 1ull);
+    if (((__ballot(!dok) >> shift) & pmask) != 0) okout = 2;
+  }
   release_v();
 }
 
