@@ -197,6 +197,15 @@ int phc_ctx_lanes_per_path(const phc_ctx *ctx, int N);
  * busiest slot works, counting units as equal). */
 int phc_ctx_counters(const phc_ctx *ctx, uint64_t *out);
 
+/* Linearization units of the last plain or batched tracking call that the
+ * grid kernels redid with the exact pivot rule and exact divisions: the
+ * fast unit screens pivots by |a|^2 and divides through a reciprocal formed
+ * once per divisor, and flags the rare unit where either does not decide
+ * bit for bit (near ties, extreme magnitudes, NaN or infinite entries).
+ * Diagnostics only: results never depend on it. No reference counterpart
+ * (the reference divides directly). */
+int phc_ctx_exact_reruns(const phc_ctx *ctx, uint64_t *out);
+
 /* Lane mapping of later tracking calls. 0 chooses per launch: an
  * autotuned choice (phc_ctx_autotune, phc_ctx_autotune_batch) for lanes
  * classes of a tuned shape, else the min(G*G, 64)-lane grid (a path's linear system one

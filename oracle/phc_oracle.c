@@ -103,6 +103,24 @@ static const double PI = 3.14159265358979323846;
 static const double HUGE_ = 1.7976931348623157e308; /* huge(1.0_wp) */
 static const double TINY_ = 2.2250738585072014e-308; /* tiny(1.0_wp) */
 
+/* Fortran MAX and MIN as flang compiles them for x86-64 (maxsd/minsd): the
+ * second argument unless the first compares greater (less). A NaN second
+ * argument is returned, a NaN first argument is not; norm_inf's fold
+ * r = max(r, abs(z(i))) is therefore NaN exactly when the last NaN component
+ * is the last component, and is otherwise the largest modulus after the
+ * last NaN. */
+static double ftn_max(double a, double b) { return a > b ? a : b; }
+static double ftn_min(double a, double b) { return a < b ? a : b; }
+/* MAXVAL: NaN elements are ignored unless every element is NaN. */
+static double maxval_fold(double r, double x, int *seen) {
+  if (x != x) return *seen ? r : x;
+  if (!*seen || r != r) {
+    *seen = 1;
+    return x;
+  }
+  return x > r ? x : r;
+}
+
 /* ------------------------------------------------- rng, src/random.f90:29-109 */
 typedef struct {
   int64_t s1, s2;
@@ -370,7 +388,7 @@ static double cond1(int n, const cx *a) { /* :135-174 */
   for (int j = 0; j < n; ++j) {
     double s = 0.0;
     for (int i = 0; i < n; ++i) s = s + cabs_(a[i + j * n]);
-    anorm = fmax(anorm, s);
+    anorm = ftn_max(anorm, s);
   }
   cx lu[65 * 65], col[65];
   int ipiv[65];
@@ -383,14 +401,14 @@ static double cond1(int n, const cx *a) { /* :135-174 */
     lu_solve(n, lu, ipiv, col);
     double s = 0.0;
     for (int i = 0; i < n; ++i) s = s + cabs_(col[i]);
-    ainv = fmax(ainv, s);
+    ainv = ftn_max(ainv, s);
   }
   return anorm * ainv;
 }
 
 static double norm_inf(int n, const cx *z) { /* :176-189 */
   double r = 0.0;
-  for (int i = 0; i < n; ++i) r = fmax(r, cabs_(z[i]));
+  for (int i = 0; i < n; ++i) r = ftn_max(r, cabs_(z[i]));
   return r;
 }
 
@@ -478,7 +496,7 @@ static void track_segment(const hom_t *h, cx *z, const seg_t_ *g,
                           const topts_t *o, info_t *info) { /* :196-264 */
   const int N = h->n + 1;
   cx zp[65];
-  double s = 0.0, step = fmin(o->step_init, o->step_max);
+  double s = 0.0, step = ftn_min(o->step_init, o->step_max);
   int nsucc = 0;
   info->naccepted = info->nrejected = 0;
   info->status = 0;
@@ -487,7 +505,7 @@ static void track_segment(const hom_t *h, cx *z, const seg_t_ *g,
       info->status = 2;
       break;
     }
-    double ds = fmin(step, 1.0 - s);
+    double ds = ftn_min(step, 1.0 - s);
     double sn = s + ds;
     if (1.0 - sn <= 1.0e-3 * step) {
       sn = 1.0;
@@ -501,7 +519,7 @@ static void track_segment(const hom_t *h, cx *z, const seg_t_ *g,
       info->naccepted++;
       nsucc++;
       if (nsucc >= o->grow_after) {
-        step = fmin(2.0 * step, o->step_max);
+        step = ftn_min(2.0 * step, o->step_max);
         nsucc = 0;
       }
     } else {
@@ -536,12 +554,13 @@ static double backward_error(const hom_t *h, const cx *z) { /* :269-285 */
   cx hv[65], hz[65 * 65], ht[65];
   hom_eval(h, z, mk(0.0, 0.0), hv, hz, ht);
   double rmax = -HUGE_;
+  int seen = 0;
   for (int i = 0; i < N; ++i) {
     double s = 0.0;
     for (int j = 0; j < N; ++j) s = s + cabs_(hz[i + j * N]);
-    rmax = fmax(rmax, s);
+    rmax = maxval_fold(rmax, s, &seen);
   }
-  return norm_inf(N, hv) / fmax(rmax * norm_inf(N, z), TINY_);
+  return norm_inf(N, hv) / ftn_max(rmax * norm_inf(N, z), TINY_);
 }
 
 static int better(double resid, double err, double best_resid, double best_err,
@@ -570,15 +589,15 @@ static int loops(const hom_t *h, const cx *z, double r, const eopts_t *eo,
       memcpy(zprev, zc, sizeof(cx) * N);
       seg_t_ g = {1, {0, 0}, {0, 0}, r, (double)(j - 1) * dtheta, dtheta};
       track_segment(h, zc, &g, so, &info);
-      so->step_init = fmin(fmax(info.step, so->step_min), 1.0);
+      so->step_init = ftn_min(ftn_max(info.step, so->step_min), 1.0);
       res->naccepted += info.naccepted;
       res->nrejected += info.nrejected;
       if (info.status != 0) return 0;
       for (int i = 0; i < N; ++i) d[i] = csub(zc[i], zprev[i]);
-      spacing = fmin(spacing, norm_inf(N, d));
+      spacing = ftn_min(spacing, norm_inf(N, d));
     }
     for (int i = 0; i < N; ++i) d[i] = csub(zc[i], z[i]);
-    if (norm_inf(N, d) <= fmax(eo->closure_factor * spacing, floor_)) {
+    if (norm_inf(N, d) <= ftn_max(eo->closure_factor * spacing, floor_)) {
       for (int i = 0; i < N; ++i) est[i] = cdivr(zsum[i], (double)nsamp);
       return 1;
     }
@@ -611,7 +630,7 @@ static void cauchy_endgame(const hom_t *h, cx *z, const eopts_t *eo,
     double err_ = HUGE_;                                                    \
     if (have_prev) {                                                        \
       for (int i = 0; i < N; ++i) d[i] = csub(est[i], prev[i]);             \
-      err_ = norm_inf(N, d) / fmax(norm_inf(N, est), TINY_);                \
+      err_ = norm_inf(N, d) / ftn_max(norm_inf(N, est), TINY_);                \
     }                                                                       \
     const double resid_ = backward_error(h, est);                           \
     if (res->status == 2 ||                                                 \
@@ -649,7 +668,7 @@ static void cauchy_endgame(const hom_t *h, cx *z, const eopts_t *eo,
     memcpy(zold, z, sizeof(cx) * N);
     seg_t_ g = {0, {r, 0.0}, {rnew, 0.0}, 0, 0, 0};
     track_segment(h, z, &g, &so, &info);
-    so.step_init = fmin(fmax(info.step, so.step_min), 1.0);
+    so.step_init = ftn_min(ftn_max(info.step, so.step_min), 1.0);
     res->naccepted += info.naccepted;
     res->nrejected += info.nrejected;
     if (info.status != 0) {
@@ -669,7 +688,7 @@ static void cauchy_endgame(const hom_t *h, cx *z, const eopts_t *eo,
       cest = -1.0;
       if (dprev > dd) cest = log(1.0 / eo->radius_factor) / log(dprev / dd);
       if (cest >= 0.5 && cest <= eo->max_winding + 0.5 && cest_prev > 0.0) {
-        if (fabs(cest - cest_prev) <= eo->agreement * fmax(1.0, cest)) {
+        if (fabs(cest - cest_prev) <= eo->agreement * ftn_max(1.0, cest)) {
           gate = 1;
           int nc = (int)lround(cest);
           max_loops = eo->max_winding < 2 * nc + 2 ? eo->max_winding : 2 * nc + 2;
@@ -781,6 +800,7 @@ static void classify(const hom_t *h, cx *z, const phc_solve_options *c,
   }
   const double zn = norm_inf(N, z);
   double rmax = -HUGE_;
+  int seen = 0;
   for (int i = 0; i < n; ++i) {
     /* abs(h(i)) / znorm**deg(i), real ** integer by repeated squaring */
     const int d = h->sys->deg[i];
@@ -792,25 +812,27 @@ static void classify(const hom_t *h, cx *z, const phc_solve_options *c,
       e >>= 1;
       if (e) pw = pw * pw;
     }
-    rmax = fmax(rmax, cabs_(hv[i]) / acc);
+    rmax = maxval_fold(rmax, cabs_(hv[i]) / acc, &seen);
   }
   q->residual = rmax;
-  const double lim = fmax(c->infinity_tol, 10.0 * fmin(p->eg_error, 1.0e-4));
+  const double lim =
+      ftn_max(c->infinity_tol, 10.0 * ftn_min(p->eg_error, 1.0e-4));
   q->at_inf = cabs_(z[n]) <= lim * zn;
   q->is_real = 0;
   if (!q->at_inf) {
     double im = -HUGE_;
+    int seen = 0;
     for (int i = 0; i < n; ++i) {
       q->x[i] = cdiv(z[i], z[n]);
-      im = fmax(im, fabs(q->x[i].im));
+      im = maxval_fold(im, fabs(q->x[i].im), &seen);
     }
-    q->is_real = im <= c->real_tol * fmax(1.0, norm_inf(n, q->x));
+    q->is_real = im <= c->real_tol * ftn_max(1.0, norm_inf(n, q->x));
   }
 }
 
 static int find(int n, const cx *x, const cx *pts, int npts, double tol) {
   /* :379-400 */
-  const double scale = fmax(1.0, norm_inf(n, x));
+  const double scale = ftn_max(1.0, norm_inf(n, x));
   cx d[64];
   for (int j = 0; j < npts; ++j) {
     for (int i = 0; i < n; ++i) d[i] = csub(x[i], pts[(size_t)j * n + i]);
@@ -965,8 +987,9 @@ int phc_orc_solve(const phc_user_system *u, const phc_solve_options *c,
   for (int j = 0; j < sm->nsolutions; ++j) {
     const cx *x = sols + (size_t)j * n;
     double im = -HUGE_;
-    for (int i = 0; i < n; ++i) im = fmax(im, fabs(x[i].im));
-    if (im <= c->real_tol * fmax(1.0, norm_inf(n, x))) {
+    int seen = 0;
+    for (int i = 0; i < n; ++i) im = maxval_fold(im, fabs(x[i].im), &seen);
+    if (im <= c->real_tol * ftn_max(1.0, norm_inf(n, x))) {
       if (sm->real_solutions)
         for (int i = 0; i < n; ++i) sm->real_solutions[(size_t)sm->nreal * n + i] = x[i].re;
       sm->nreal++;

@@ -167,8 +167,8 @@ struct Launch {
   int32_t resume;           // load saved states at entry
   int32_t max_units;        // units per group per launch, 0 = unlimited
   unsigned int* active;     // groups still holding a path at exit
-  unsigned long long* counters;  // [4]: reserved, units, unit slots, most
-                                 // units run by one path slot
+  unsigned long long* counters;  // [4]: units redone exactly, units, unit
+                                 // slots, most units run by one path slot
 };
 
 // ---------------------------------------------------------------- group ops
@@ -234,10 +234,63 @@ PHC_D void gargmax(double& v, int& p, int& l) {
   argmax_step<DPP_XOR2>(v, p, l);
 }
 
-// max_i |v_i| over the N components held by the group (src/linalg.f90:176).
+// A ballot restricted to the calling lane's G-lane group (bit i: lane i of
+// the group).
+template <int G>
+PHC_D unsigned long long group_ballot(bool p) {
+  const unsigned long long b = __ballot(p);
+  if constexpr (G == 64) return b;
+  return (b >> ((__lane_id() / G) * G)) & ((1ull << G) - 1ull);
+}
+
+// norm_inf (src/linalg.f90:176-189) of the moduli `a` of the N components
+// held by the group (lane < N; 0 elsewhere): the fold r = max(r, a_i) from
+// r = 0 with Fortran MAX (ftn_max). Without NaNs that is the largest
+// modulus; with NaNs it is NaN when the last component is NaN and otherwise
+// the largest modulus after the last NaN component.
+template <int G>
+PHC_D double gnorm_abs(double a, int lane, int N) {
+  const unsigned long long nan = group_ballot<G>(lane < N && a != a);
+  const int last = nan ? 63 - __clzll(nan) : -1;
+  const double r = gmax<G>(lane > last ? a : 0.0);
+  return last == N - 1 ? __longlong_as_double(0x7ff8000000000000ll) : r;
+}
 template <int G>
 PHC_D double gnorm(cx v, int lane, int N) {
-  return gmax<G>(lane < N ? cabs_(v) : 0.0);
+  return gnorm_abs<G>(lane < N ? cabs_(v) : 0.0, lane, N);
+}
+
+// MAXVAL over the values `a` of the group's lanes < N: NaNs are ignored
+// unless every value is NaN (flang's MAXVAL). Non-negative values only
+// (row sums of moduli): other lanes contribute 0.
+template <int G>
+PHC_D double gmaxval_nonneg(double a, int lane, int N) {
+  const unsigned long long nan = group_ballot<G>(lane < N && a != a);
+  const double r = gmax<G>(lane < N && a == a ? a : 0.0);
+  return nan == (N >= 64 ? ~0ull : (1ull << N) - 1ull)
+             ? __longlong_as_double(0x7ff8000000000000ll)
+             : r;
+}
+
+// The pivot of column k (src/linalg.f90:51-58) among the group's candidate
+// rows, `av` = |a(i, k)| of the row at logical position pos: the scan starts
+// at position k and takes a later row only if its modulus is strictly
+// larger, so a NaN at position k stays the pivot and a later NaN never
+// wins; ties go to the smaller position. bv, bp, bl: the pivot's modulus,
+// position and lane in the group.
+template <int G>
+PHC_D void pivot_argmax(bool cand, double av, int pos, int k, int lane,
+                        double& bv, int& bp, int& bl) {
+  bv = (cand && av == av) ? av : -1.0;
+  bp = pos;
+  bl = lane;
+  gargmax<G>(bv, bp, bl);
+  const unsigned long long nk = group_ballot<G>(cand && pos == k && av != av);
+  if (nk) {
+    bv = __longlong_as_double(0x7ff8000000000000ll);
+    bp = k;
+    bl = __ffsll(nk) - 1;
+  }
 }
 
 // A pointer read from memory (a request's tables) is generic to the
@@ -525,14 +578,14 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
 
   if (mode == PC_BE) {
     // ||H(z,0)|| / max(max_i sum_j |H_z(i,j)| * ||z||, tiny)
-    const double hn = gmax<G>(row ? cabs_(h) : 0.0);
+    const double hn = gnorm<G>(h, lane, N);
     double rs = 0.0;
 #pragma unroll
     for (int j = 0; j < G; ++j)
       if (j < N) rs = rs + cabs_(hz[j]);
-    const double rmax = gmax<G>(row ? rs : 0.0);
+    const double rmax = gmaxval_nonneg<G>(rs, lane, N);
     const double zn = gnorm<G>(zq, lane, N);
-    be = hn / fmax(rmax * zn, DBL_MIN);
+    be = hn / ftn_max(rmax * zn, DBL_MIN);
     okout = 1;
     xout = mk(0.0, 0.0);
     return;
@@ -556,9 +609,9 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
   wave_lds_fence();
   for (int k = 0; k < N; ++k) {
     const cx ak = row ? arow[k] : mk(0.0, 0.0);
-    double bv = (row && pos >= k) ? cabs_(ak) : -1.0;
-    int bp = pos, bl = lane;
-    gargmax<G>(bv, bp, bl);
+    double bv;
+    int bp, bl;
+    pivot_argmax<G>(row && pos >= k, cabs_(ak), pos, k, lane, bv, bp, bl);
     if (bv == 0.0) {
       ok = 0;
       break;
@@ -677,7 +730,7 @@ struct Machine {
     S.sdth = kind ? R.dtheta : 0.0;
     S.ret = ret;
     S.s = 0.0;
-    S.step = fmin(step_init, step_max);
+    S.step = ftn_min(step_init, step_max);
     S.o_step_max = step_max;
     S.o_grow = grow;
     S.nsucc = 0;
@@ -698,7 +751,7 @@ struct Machine {
       S.pc = X_SEG_FINISH;
       return;
     }
-    S.ds = fmin(S.step, 1.0 - S.s);
+    S.ds = ftn_min(S.step, 1.0 - S.s);
     S.sn = S.s + S.ds;
     if (1.0 - S.sn <= 1.0e-3 * S.step) {
       S.sn = 1.0;
@@ -717,7 +770,7 @@ struct Machine {
     S.sacc += 1;
     S.nsucc += 1;
     if (S.nsucc >= S.o_grow) {
-      S.step = fmin(2.0 * S.step, S.o_step_max);
+      S.step = ftn_min(2.0 * S.step, S.o_step_max);
       S.nsucc = 0;
     }
     S.pc = X_SEG_HEAD;
@@ -891,7 +944,7 @@ struct Machine {
 
   PHC_D void eg_after_radial() {
     zeg() = zt();
-    S.seg_init = fmin(fmax(S.step, R.step_min), 1.0);
+    S.seg_init = ftn_min(ftn_max(S.step, R.step_min), 1.0);
     S.eg_acc += S.sacc;
     S.eg_rej += S.srej;
     if (S.sstatus != PHC_PATH_SUCCESS) {
@@ -913,7 +966,7 @@ struct Machine {
       if (S.dprev > d) S.cest = R.log_inv_rf / log(S.dprev / d);
       if (S.cest >= 0.5 && S.cest <= (double)R.max_winding + 0.5 &&
           S.cest_prev > 0.0) {
-        if (fabs(S.cest - S.cest_prev) <= R.agreement * fmax(1.0, S.cest)) {
+        if (fabs(S.cest - S.cest_prev) <= R.agreement * ftn_max(1.0, S.cest)) {
           S.gate = 1;
           S.max_loops = min(R.max_winding, 2 * (int)round(S.cest) + 2);
         }
@@ -937,7 +990,7 @@ struct Machine {
   PHC_D void record_begin() {
     S.rec_err = DBL_MAX;
     if (S.have_prev)
-      S.rec_err = norm(csub(est(), prev())) / fmax(norm(est()), DBL_MIN);
+      S.rec_err = norm(csub(est(), prev())) / ftn_max(norm(est()), DBL_MIN);
     zq() = est();
     S.pc = PC_BE;
   }
@@ -1001,7 +1054,7 @@ struct Machine {
   }
 
   PHC_D void loops_after_arc() {
-    S.seg_init = fmin(fmax(S.step, R.step_min), 1.0);
+    S.seg_init = ftn_min(ftn_max(S.step, R.step_min), 1.0);
     S.eg_acc += S.sacc;
     S.eg_rej += S.srej;
     if (S.sstatus != PHC_PATH_SUCCESS) {
@@ -1009,13 +1062,13 @@ struct Machine {
       S.pc = X_LOOPS_END;
       return;
     }
-    S.spacing = fmin(S.spacing, norm(csub(zt(), zprev())));
+    S.spacing = ftn_min(S.spacing, norm(csub(zt(), zprev())));
     S.lj += 1;
     if (S.lj <= R.nsamples) {
       S.pc = X_LOOPS_ARC;
       return;
     }
-    if (norm(csub(zt(), zeg())) <= fmax(R.closure * S.spacing, S.floor)) {
+    if (norm(csub(zt(), zeg())) <= ftn_max(R.closure * S.spacing, S.floor)) {
       est() = cdiv(zsum(), mk((double)S.nsamp, 0.0));
       S.lok = 1;
       S.pc = X_LOOPS_END;
@@ -1277,9 +1330,12 @@ __global__ void __launch_bounds__(WG)
                               dt, wl, x, ok, be);
         }
         if (ok == 2) {
-          // the fast LU's screen was inconclusive: the same unit with the
-          // exact pivot rule in every step (inputs read again from LDS, so
-          // none of them stays in registers through the first unit)
+          // the fast LU's screen was inconclusive, or a quotient fell
+          // outside the fast division's range: the same unit with the exact
+          // pivot rule and divisions in every step (inputs read again from
+          // LDS, so none of them stays in registers through the first unit)
+          if (writer && lane == 0 && L.counters)
+            atomicAdd(&L.counters[0], 1ull);
           __asm__ __volatile__("" ::: "memory");
           cx t, dt, zl;
           inputs(t, dt, zl);

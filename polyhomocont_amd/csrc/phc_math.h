@@ -68,6 +68,14 @@ PHC_HD cx cmul(cx a, cx b) {
 // (s, 0) * a for a real (or integer-valued) scalar s.
 PHC_HD cx cscale(double s, cx a) { return mk(s * a.re, s * a.im); }
 
+// Fortran MAX and MIN as flang compiles them for x86-64 (maxsd, minsd):
+// the second argument unless the first compares greater (less). A NaN
+// second argument is returned, a NaN first argument is not -- unlike
+// fmax/fmin, which drop NaNs. Paths that overflow to NaN take the
+// reference's steps only with these.
+PHC_HD double ftn_max(double a, double b) { return a > b ? a : b; }
+PHC_HD double ftn_min(double a, double b) { return a < b ? a : b; }
+
 // C99 Annex G recovery of infinities and zeros for __divdc3 when the
 // formula gave NaN + i NaN (c, d already scaled by 2^-ilogbw). The divisor
 // was infinite (logb = +inf) exactly when c or d is: a finite divisor's

@@ -46,6 +46,7 @@ struct phc_ctx {
   // phc_ctx_autotune_batch)
   std::map<std::array<int64_t, 4>, std::array<int, 3>> tuned;
   unsigned long long counters[4] = {0, 0, 0, 0};  // of the last run
+  unsigned long long reruns = 0;  // units redone exactly in the last run
   // launch shape of every lanes class of the last run (phc_ctx_last_plan)
   std::vector<std::array<int32_t, 8>> last_plan;
   std::mutex mu;
@@ -799,11 +800,13 @@ int fetch_array(phc_ctx* ctx, T* host, const T* dev, size_t count) {
 
 int fetch_outputs(phc_ctx* ctx, phc_staged* st) {
   for (int i = 0; i < 4; ++i) ctx->counters[i] = 0;
+  ctx->reruns = 0;
   for (ClassRun& cr : st->classes) {
     unsigned long long c[4];
     HIPCHK(ctx, hipMemcpyAsync(c, cr.d_counters, sizeof(c),
                                hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->reruns += c[0];       // lane 0 of a group adds
     ctx->counters[1] += c[1];  // lane 0 of a group adds
     ctx->counters[2] += c[2];  // lane 0 of a wavefront adds
     ctx->counters[3] = std::max<unsigned long long>(ctx->counters[3], c[3]);
@@ -992,6 +995,12 @@ int phc_get_default_device(void) { return g_default_device.load(); }
 int phc_ctx_counters(const phc_ctx* ctx, uint64_t* out) {
   if (!ctx || !out) return PHC_ERR_INVALID_INPUT;
   for (int i = 0; i < 4; ++i) out[i] = ctx->counters[i];
+  return PHC_OK;
+}
+
+int phc_ctx_exact_reruns(const phc_ctx* ctx, uint64_t* out) {
+  if (!ctx || !out) return PHC_ERR_INVALID_INPUT;
+  *out = ctx->reruns;
   return PHC_OK;
 }
 
@@ -1440,6 +1449,7 @@ int phc_track_paths_noalloc(phc_ctx* ctx, const phc_system* sys,
   ctx->counters[1] = c[1];
   ctx->counters[2] = c[2];
   ctx->counters[3] = c[3];
+  ctx->reruns = c[0];
   return PHC_OK;
 }
 

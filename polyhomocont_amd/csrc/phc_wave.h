@@ -543,9 +543,9 @@ PHC_D void jac_general(const Req& R, WP wt,
 // the largest too (|a|^2 and hypot are both within a few units in the last
 // place), so the hypot values are not needed. Near ties, NaN entries and
 // |a|^2 outside the normal range fall back to comparing the hypot values
-// themselves.
+// themselves, with the reference's rule for NaN moduli (pivot_argmax).
 template <int G, int LP>
-PHC_D void pivot_search(bool cand, cx ak, int pos, int lane, int src,
+PHC_D void pivot_search(bool cand, cx ak, int pos, int k, int lane, int src,
                         double& bv, int& bp, int& bl) {
   const double sq = ak.re * ak.re + ak.im * ak.im;
   double sv = cand ? sq : -1.0;
@@ -562,10 +562,7 @@ PHC_D void pivot_search(bool cand, cx ak, int pos, int lane, int src,
     bl = pbcast_i<LP>(sl, src);
     return;
   }
-  bv = cand ? cabs_(ak) : -1.0;
-  bp = pos;
-  bl = lane;
-  gargmax<G>(bv, bp, bl);
+  pivot_argmax<G>(cand, cabs_(ak), pos, k, lane, bv, bp, bl);
   bv = pbcast_d<LP>(bv, src);
   bp = pbcast_i<LP>(bp, src);
   bl = pbcast_i<LP>(bl, src);
@@ -598,8 +595,8 @@ PHC_D int lu_slots(int N, cx (&a)[GridShape<G>::E],
         const cdiv_den mine = cdiv_scale(a[ke]);
         double bv;
         int bp, bl;
-        pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, lane,
-                            kr * G, bv, bp, bl);
+        pivot_search<G, LP>(in[ke] && rep == kr && pos >= k, a[ke], pos, k,
+                            lane, kr * G, bv, bp, bl);
         if (bv == 0.0) {
           ok = 0;
           break;
@@ -648,8 +645,8 @@ PHC_D int lu_one_slot_exact(int N, cx& a, const bool in, const int col,
     const cdiv_den mine = cdiv_scale(a);  // the winner's is the pivot's
     double bv;
     int bp, bl;
-    pivot_search<G, LP>(in && rep == k && pos >= k, a, pos, lane, k * G, bv,
-                        bp, bl);
+    pivot_search<G, LP>(in && rep == k && pos >= k, a, pos, k, lane, k * G,
+                        bv, bp, bl);
     if (bv == 0.0) {
       ok = 0;
       break;
@@ -1052,10 +1049,10 @@ PHC_D void unit_wave(const Req& R, WP wt,
       hv = cabs_(W.H[lane]);
       for (int j = 0; j < N; ++j) rs = rs + cabs_(W.A[lane * G + j]);
     }
-    const double hn = gmax<G>(hv);
-    const double rmax = gmax<G>(rs);
+    const double hn = gnorm_abs<G>(hv, lane, N);
+    const double rmax = gmaxval_nonneg<G>(rs, lane, N);
     const double zn = gnorm<G>(zq, lane, N);
-    be = hn / fmax(rmax * zn, DBL_MIN);
+    be = hn / ftn_max(rmax * zn, DBL_MIN);
     okout = 1;
     xout = mk(0.0, 0.0);
     release_v();

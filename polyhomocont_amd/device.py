@@ -44,6 +44,7 @@ def lib():
         L.phc_ctx_last_error.restype = C.c_char_p
         L.phc_ctx_sync.argtypes = [P]
         L.phc_ctx_counters.argtypes = [P, P]
+        L.phc_ctx_exact_reruns.argtypes = [P, P]
         L.phc_ctx_set_mapping.argtypes = [P, C.c_int, C.c_int]
         L.phc_ctx_autotune.argtypes = [P, P, P, P, i64, P, P, P]
         L.phc_ctx_clear_tuning.argtypes = [P]
@@ -118,6 +119,14 @@ class Context:
         return dict(units=int(c[1]), unit_slots=int(c[2]), occupancy=occ,
                     slots=int(c[0]), max_slot_units=int(c[3]),
                     slot_utilisation=util)
+
+    def exact_reruns(self):
+        """Units of the last tracking call redone with the exact pivot rule
+        and divisions (phc_ctx_exact_reruns)."""
+        c = np.zeros(1, np.uint64)
+        self._check(self.L.phc_ctx_exact_reruns(self.h, c.ctypes.data),
+                    "phc_ctx_exact_reruns")
+        return int(c[0])
 
     def set_mapping(self, mapping=0, wg_threads=0):
         """0: chosen per launch, 1: path per G-lane group, 2: path per

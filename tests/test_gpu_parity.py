@@ -86,6 +86,47 @@ def test_c3_sample_bit_exact(ctx, ref, name, first, count):
     assert_bit_exact(got, want, count)
 
 
+def scaled_system(usys, s):
+    return abi.UserSystemData(usys.n, [(c * s, e) for c, e in usys.equations])
+
+
+# The fast unit divides through a reciprocal formed once per divisor, which
+# is the compiler's own division only for numerators of magnitude in
+# [2^-969, 2^768); outside it (and for NaN or infinite entries) the unit is
+# redone with exact divisions. Coefficients scaled by 2^790 or 2^-990 (exact
+# scalings) put the LU's and solves' numerators there.
+@pytest.mark.parametrize("name", ["c1", "sparse_n8"])
+@pytest.mark.parametrize("scale", [2.0 ** 790, 2.0 ** -990])
+def test_extreme_magnitudes_redone_exactly(ctx, ref, name, scale):
+    base = W.config_system("c1") if name == "c1" else W.random_sparse(170)
+    assert name == "c1" or base.n == 8  # the 16-lane grid
+    usys = scaled_system(base, scale)
+    opts = abi.default_options()
+    g, b, a, ns = ref.homotopy(usys, opts.seed)
+    cnt = min(ns, 64)
+    start = ref.start_points(usys, opts.seed, 0, cnt)
+    print(f"[gpu] {name} x {scale:.3e}: {cnt} paths", flush=True)
+    got = run_device(ctx, usys, opts, (g, b, a), start)
+    assert ctx.exact_reruns() > 0
+    want = ref.endpoints(usys, opts, 0, cnt)
+    for f in FIELDS + ("eg_error", "t"):
+        assert np.array_equal(getattr(got, f)[:cnt], getattr(want, f)[:cnt],
+                              equal_nan=True), f
+    assert np.array_equal(got.z[:cnt], want.z[:cnt], equal_nan=True)
+
+
+def test_ordinary_units_rarely_redone(ctx, ref):
+    usys = W.config_system("c2")
+    opts = abi.default_options()
+    g, b, a, ns = ref.homotopy(usys, opts.seed)
+    cnt = 256
+    start = ref.start_points(usys, opts.seed, 0, cnt)
+    got = run_device(ctx, usys, opts, (g, b, a), start)
+    units = ctx.counters()["units"]
+    assert ctx.exact_reruns() <= units // 1000, (ctx.exact_reruns(), units)
+    assert_bit_exact(got, ref.endpoints(usys, opts, 0, cnt), cnt)
+
+
 def test_random_sparse_batch_bit_exact(ctx, ref):
     """Random sparse systems with n = 3, 4, 5, 7, 8 (2302 paths) in one
     batched call, every path against the reference."""

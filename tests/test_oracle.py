@@ -126,6 +126,29 @@ def test_c2_c3_samples_match_reference(port, ref):
         assert np.array_equal(a.z[:count], b.z[:count])
 
 
+# Systems scaled by powers of two far from 1 overflow while tracking: NaN
+# components then meet the reference's MAX / MIN (flang: the second
+# argument unless the first compares greater / less), e.g. in norm_inf's
+# fold, and at 2^790 the reference fails every C1 path at the first step.
+# IEEE fmax / fmin there give other step sequences.
+@pytest.mark.parametrize("name", ["c1", "sparse_n8"])
+@pytest.mark.parametrize("scale", [2.0 ** 790, 2.0 ** -990, 2.0 ** 600])
+def test_overflowing_paths_match_reference(port, ref, name, scale):
+    base = W.config_system("c1") if name == "c1" else W.random_sparse(170)
+    usys = abi.UserSystemData(base.n, [(c * scale, e) for c, e in base.equations])
+    opts = abi.default_options()
+    cnt = min(usys.bezout, 48)
+    a = port.endpoints(usys, opts, 0, cnt)
+    b = ref.endpoints(usys, opts, 0, cnt)
+    for f in ("status", "winding", "eg_status", "naccepted", "nrejected", "t",
+              "eg_error"):
+        assert np.array_equal(getattr(a, f)[:cnt], getattr(b, f)[:cnt],
+                              equal_nan=True), f
+    assert np.array_equal(a.z[:cnt], b.z[:cnt], equal_nan=True)
+    if name == "c1" and scale > 1e200:
+        assert (b.status[:cnt] == 1).all()  # the case that tells max from fmax
+
+
 @pytest.mark.parametrize("name", SMALL)
 def test_solve_matches_reference(port, ref, name):
     st1, a = port.solve(S[name], abi.default_options())
