@@ -65,6 +65,9 @@ struct alignas(16) Req {
                      // 0-7), of a job's prefix (8-15) and suffix (16-23),
                      // product length of the short job layout (24-27)
   int32_t vstride;   // grid unit: V entries per equation
+  int32_t vtail;     // grid unit: V entries after the equations' (four
+                     // zeros; for the one-slot general layout also a one
+                     // and the terms' prefix and suffix products)
   int32_t dbits;     // bits of the largest deg[i] - 1 (start-system powers)
   int32_t first[PHC_MAX_UNKNOWNS + 1];
   int32_t deg[PHC_MAX_UNKNOWNS];

@@ -303,6 +303,7 @@ bool build_runs(HostReq& h, int G, std::vector<WTerm>& wt,
   h.wterms.swap(wt);
   r.njobs = (int32_t)M;
   r.vstride = vs;
+  r.vtail = 4;
   r.gfold = maxnnz | (maxp << 8) | (maxs << 16) | (fl << 24) | (1 << 28);
   return true;
 }
@@ -379,9 +380,32 @@ bool build_grid(HostReq& h, int G) {
   // padding rows: the zero-coefficient term with no other factors (adds
   // +0), except for the general layout of the 16-lane grid, whose loop
   // skips WJOB_NONE rows
-  const uint4 pad = (fl || E == 1)
-                        ? make_uint4((uint32_t)nt | (1u << 20), 0u, 0u, 0u)
-                        : make_uint4(WJOB_NONE, 0u, 0u, 0u);
+  // The general layout with one slot per lane keeps each term's prefix
+  // products P_1..P_{m-1} and suffix products S_2..S_m (m nonzero factors)
+  // in V after the four zeros and a one: term k's from ps_base[k] on,
+  // prefixes first (WTerm::nnz bits 8 and up). A job record then names
+  // them (z = V index of P_{q-1} | of S_{q+1} << 16; the one where a
+  // product is empty) and the factor pw(e-1, j) (y).
+  const bool ps_layout = !fl && E == 1;
+  const int one_slot = n * vstride + 4;
+  std::vector<int> ps_base(nt + 1, 0);
+  int vtail = 4;
+  if (ps_layout) {
+    int ps = one_slot + 1;
+    for (int k = 0; k < nt; ++k) {
+      ps_base[k] = ps;
+      const int nz = (int)wt[k].nnz;
+      wt[k].nnz = (uint32_t)nz | ((uint32_t)ps << 8);
+      ps += 2 * std::max(nz - 1, 0);
+    }
+    if (ps > 65535) return false;  // 16-bit indices in the job records
+    vtail = ps - n * vstride;
+  }
+  const uint4 pad =
+      ps_layout ? make_uint4((uint32_t)nt | (1u << 20), 0u,
+                             (uint32_t)one_slot | ((uint32_t)one_slot << 16), 0u)
+      : fl      ? make_uint4((uint32_t)nt | (1u << 20), 0u, 0u, 0u)
+                : make_uint4(WJOB_NONE, 0u, 0u, 0u);
   h.jobs.assign(M * LP * E, pad);
   for (int qq = 0; qq < LP * E; ++qq) {
     for (size_t m = 0; m < lists[qq].size(); ++m) {
@@ -389,6 +413,13 @@ bool build_grid(HostReq& h, int G) {
       uint4 rec = make_uint4((uint32_t)jb.k | ((uint32_t)jb.e << 20), 0u, 0u, 0u);
       if (fl) {
         rec = short_record(jb);
+      } else if (ps_layout) {
+        // the job of the term's factor a = np (0-based) of nz = np + ns + 1
+        const int a = jb.np, nz = jb.np + jb.ns + 1, b = ps_base[jb.k];
+        const int pre = a >= 1 ? b + (a - 1) : one_slot;
+        const int suf = a <= nz - 2 ? b + (nz - 1) + a : one_slot;
+        rec.y = (uint32_t)jb.base;
+        rec.z = (uint32_t)pre | ((uint32_t)suf << 16);
       } else {
         uint64_t offs = (uint64_t)jb.base;
         for (int b2 = 0; b2 < jb.np + jb.ns; ++b2)
@@ -403,6 +434,7 @@ bool build_grid(HostReq& h, int G) {
   h.wterms.swap(wt);
   r.njobs = (int32_t)M;
   r.vstride = vstride;
+  r.vtail = vtail;
   r.gfold = maxnnz | (maxp << 8) | (maxs << 16) | (fl << 24);
   return true;
 }
@@ -460,8 +492,9 @@ size_t split_lds_bytes(int G) {
   return (R * G * (G + 1) + (size_t)G * G + G) * sizeof(cx);
 }
 // Entries of a path's term-value array V for a request's grid tables: its
-// n equations at vstride entries each, then the four zeros.
-int grid_v_entries(const Req& r) { return r.n * r.vstride + 4; }
+// n equations at vstride entries each, then the four zeros (and, for the
+// one-slot general layout, the one and the prefix and suffix products).
+int grid_v_entries(const Req& r) { return r.n * r.vstride + r.vtail; }
 
 size_t grid_lds_bytes(int G, int expmax, int vmax) {
   return state_lds_bytes(G) + wave_lds_entries(G, expmax, vmax) * sizeof(cx);

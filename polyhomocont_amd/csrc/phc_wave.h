@@ -281,6 +281,39 @@ PHC_D void term_values(const Req& R, WP wt,
   }
 }
 
+// Term values of the one-slot general layout (jac_general_ps), which
+// also keep the term's prefix products P_a = f_1 ... f_a (a < m, left fold)
+// and suffix products S_b = f_b (f_{b+1} ( ... f_m)) (2 <= b <= m, right
+// fold) at V[ps + a - 1] and V[ps + m - 1 + b - 2], ps = WTerm::nnz >> 8
+// (src/sparse.f90:259-267). Factors past the m-th are pw(0, 0) = 1, as in
+// term_value. Lane 0 of the path writes the one that empty products name.
+template <int LP, class WP>
+PHC_D void term_values_ps(const Req& R, WP wt, const cx* __restrict__ T,
+                          cx* __restrict__ V, int q, int one_slot) {
+  if (q == 0) V[one_slot] = mk(1.0, 0.0);
+  for (int k = q; k < R.nterms; k += LP) {
+    const WTerm w = ld(wt, k);
+    const int m = (int)(w.nnz & 255u), ps = (int)(w.nnz >> 8);
+    cx f[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) f[a] = T[woff(w.off, a)];
+    cx P = f[0];
+#pragma unroll
+    for (int a = 1; a < 8; ++a) {
+      if (a < m) V[ps + a - 1] = P;  // P_a
+      P = cmul(P, f[a]);
+    }
+    V[w.slot] = cmul(mk(w.re, w.im), P);
+    cx S = f[7];  // S_8
+    if (m == 8) V[ps + 7 + 6] = S;
+#pragma unroll
+    for (int b = 7; b >= 2; --b) {
+      S = cmul(f[b - 1], S);  // S_b
+      if (b <= m) V[ps + m - 1 + b - 2] = S;
+    }
+  }
+}
+
 // The coefficient of a job record's term.
 template <class WP>
 PHC_D cx job_coef(const uint4 job, WP wt) {
@@ -424,15 +457,19 @@ PHC_D cx job_product_general(const uint4 job, const cx c,
   return cmul(cmul(d, P), S);
 }
 
-// General job layout with one slot per lane: the loop of jac_short (two
-// jobs per turn, next turn's records prefetched, four term values per
-// step) around job_product_general. Rows past a lane's last job are the
-// zero-coefficient term with no other factors, so they add +0.
-template <int G, int FLG, class WP, class JP>
-PHC_D void jac_general1(const Req& R, WP wt,
-                        JP jobs,
-                        const cx* __restrict__ T, const cx* __restrict__ V,
-                        int q, int vbase, int vstep, cx& acc) {
+// General job layout with one slot per lane (G = 4, 8; terms with more
+// than four nonzero exponents). The term pass (term_values_ps) keeps each
+// term's prefix products P_1..P_{m-1} and suffix products S_2..S_m in V,
+// so the job of the term's q-th factor, c e pw(e-1, j) pre(j-1) suf(j+1)
+// (src/sparse.f90:271-273), is ((c e) pw(e-1, j)) P_{q-1} S_{q+1}: three
+// products, in the reference's association (an empty product is the one
+// in V). The loop of jac_short around it: two jobs per turn, the next
+// turn's records prefetched, four term values per step for the F lanes.
+// Rows past a lane's last job are the zero-coefficient term, adding +0.
+template <int G, class WP, class JP>
+PHC_D void jac_general_ps(const Req& R, WP wt, JP jobs,
+                          const cx* __restrict__ T, const cx* __restrict__ V,
+                          int q, int vbase, int vstep, cx& acc) {
   constexpr int LP = GridShape<G>::LP, U = 2;
   const int M = R.njobs;
   acc = mk(0.0, 0.0);
@@ -450,7 +487,12 @@ PHC_D void jac_general1(const Req& R, WP wt,
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const cx d = job_product_general<FLG>(jc[u], job_coef(jc[u], wt), T);
+      const uint4 j = jc[u];
+      const cx c = job_coef(j, wt);
+      const cx B = T[j.y];
+      const cx pre = V[j.z & 0xFFFFu], suf = V[j.z >> 16];
+      const int ex = (int)((j.x >> 20) & 255u);
+      const cx d = cmul(cmul(cmul(cscale((double)ex, c), B), pre), suf);
       const int vb = vbase + vstep * (m + u);
       const cx v0 = V[vb], v1 = V[vb + 1], v2 = V[vb + 2], v3 = V[vb + 3];
       acc = cadd(acc, d);
@@ -460,23 +502,6 @@ PHC_D void jac_general1(const Req& R, WP wt,
       acc = cadd(acc, v3);
     }
   }
-}
-
-// jac_general1 for FLG = max(maxp, maxs) among NC .. 7 (a term has at
-// most 8 nonzero exponents, so a job has at most 7 other factors).
-template <int G, int NC, class WP, class JP>
-PHC_D void jac_general1_dispatch(int flg, const Req& R, WP wt, JP jobs,
-                                 const cx* __restrict__ T,
-                                 const cx* __restrict__ V, int q, int vbase,
-                                 int vstep, cx& acc) {
-  if constexpr (NC < 7) {
-    if (flg > NC) {
-      jac_general1_dispatch<G, NC + 1>(flg, R, wt, jobs, T, V, q, vbase,
-                                       vstep, acc);
-      return;
-    }
-  }
-  jac_general1<G, NC>(R, wt, jobs, T, V, q, vbase, vstep, acc);
 }
 
 // The same with the general job layout (any number of factors up to 8).
@@ -534,6 +559,14 @@ PHC_D void jac_general(const Req& R, WP wt,
     }
     for (; fk < vcount[e]; ++fk) acc[e] = cadd(acc[e], V[vbase[e] + fk]);
   });
+}
+
+// The |a|^2 screen of the fast LUs decides only for a largest |a|^2 in
+// [2^-930, 2^930), well inside the normal range: a test of its exponent
+// field (a negative value or NaN fails it).
+PHC_D bool screen_range(double sv) {
+  const unsigned int hi = (unsigned int)((unsigned long long)__double_as_longlong(sv) >> 32);
+  return (hi >> 20) - (1023u - 930u) < 1860u;
 }
 
 // Pivot of one LU step among the candidates `cand` of the column's group
@@ -699,11 +732,14 @@ PHC_D int lu_one_slot(int N, cx& a, const bool in, const int col, int rep,
     const cdiv_den mine = cdiv_scale(a);  // the winner's is the pivot's
     const double sq = a.re * a.re + a.im * a.im;
     const double sv = pbcast_d<LP>(gmax<G>(cand ? sq : -1.0), gbase);
-    const unsigned long long hit = (__ballot(cand && sq == sv) >> shift) & pmask;
+    const unsigned long long hit =
+        (__builtin_amdgcn_ballot_w64(cand && sq == sv) >> shift) & pmask;
     const unsigned long long near =
-        (__ballot(cand && (sq >= sv * (1.0 - 1e-12) || sq != sq)) >> shift) &
+        (__builtin_amdgcn_ballot_w64(cand &&
+                                     (sq >= sv * (1.0 - 1e-12) || sq != sq)) >>
+         shift) &
         pmask;
-    sure = sure && sv > 1e-280 && sv < 1e280 && __popcll(near) == 1;
+    sure = sure && screen_range(sv) && __popcll(near) == 1;
     // the winner's lane in the path (any lane of the group if none: the
     // step is then redone exactly)
     const int wl = hit ? __ffsll(hit) - 1 : gbase;
@@ -759,10 +795,11 @@ PHC_D int lu_slots_fast(int N, cx (&a)[GridShape<G>::E],
         const cdiv_den mine = cdiv_scale(a[ke]);  // the winner's: the pivot's
         const double sq = a[ke].re * a[ke].re + a[ke].im * a[ke].im;
         const double sv = pbcast_d<LP>(gmax<G>(cand ? sq : -1.0), gbase);
-        const unsigned long long hit = __ballot(cand && sq == sv);
-        const unsigned long long near =
-            __ballot(cand && (sq >= sv * (1.0 - 1e-12) || sq != sq));
-        sure = sure && sv > 1e-280 && sv < 1e280 && __popcll(near) == 1;
+        const unsigned long long hit =
+            __builtin_amdgcn_ballot_w64(cand && sq == sv);
+        const unsigned long long near = __builtin_amdgcn_ballot_w64(
+            cand && (sq >= sv * (1.0 - 1e-12) || sq != sq));
+        sure = sure && screen_range(sv) && __popcll(near) == 1;
         const int wl = hit ? __ffsll(hit) - 1 : gbase;
         const int bl = wl - gbase;
         const int bp = pbcast_i<LP>(pos, wl);
@@ -917,6 +954,8 @@ PHC_D void unit_wave(const Req& R, WP wt,
   // 2. term values
   if (fl_len != 0)
     term_values<LP, 4>(R, wt, T, V, q);
+  else if constexpr (E == 1)
+    term_values_ps<LP>(R, wt, T, V, q, R.n * R.vstride + 4);
   else
     term_values<LP, 8>(R, wt, T, V, q);
   wave_lds_fence();
@@ -963,10 +1002,7 @@ PHC_D void unit_wave(const Req& R, WP wt,
     else if (fl_len == 3)
       jac_short<G, 3>(R, wt, jobs, T, V, q, vbase[0], vstep, acc[0]);
     else
-      jac_general1_dispatch<G, 1>(
-          gf_maxp(R.gfold) > gf_maxs(R.gfold) ? gf_maxp(R.gfold)
-                                              : gf_maxs(R.gfold),
-          R, wt, jobs, T, V, q, vbase[0], vstep, acc[0]);
+      jac_general_ps<G>(R, wt, jobs, T, V, q, vbase[0], vstep, acc[0]);
   }
 
   // 4. homotopy (src/homotopy.f90:491-511) and the patch row; the slot
