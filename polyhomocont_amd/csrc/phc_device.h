@@ -638,19 +638,15 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
     return;
   }
 
-  // 5. forward substitution with unit L in logical row order
+  // 5. forward substitution with unit L in logical row order; Fortran's
+  // SUM starts from +0
   cx sacc = mk(0.0, 0.0);
-  bool sset = false;
   for (int k = 0; k < N; ++k) {
-    const cx cand = (pos == 0 || !sset) ? b : csub(b, sacc);
+    const cx cand = pos == 0 ? b : csub(b, sacc);
     const cx yk = gshfl_c<G>(cand, L.PL[k]);
-    if (row && pos > k) {
-      const cx p = cmul(arow[k], yk);
-      sacc = sset ? cadd(sacc, p) : p;
-      sset = true;
-    }
+    if (row && pos > k) sacc = cadd(sacc, cmul(arow[k], yk));
   }
-  const cx yfin = (pos == 0 || !sset) ? b : csub(b, sacc);
+  const cx yfin = pos == 0 ? b : csub(b, sacc);
   // back substitution, sums in increasing column order; the row at
   // logical position i is owned by lane PL[i]
   cx* __restrict__ X = L.X;
@@ -660,7 +656,7 @@ PHC_D void unit(const Req& R, const TermRec* __restrict__ terms,
     if (row) {
       cx num = yfin;
       if (i + 1 < N) {
-        cx s = cmul(arow[i + 1], X[i + 1]);
+        cx s = cadd(mk(0.0, 0.0), cmul(arow[i + 1], X[i + 1]));
         for (int j = i + 2; j < N; ++j) s = cadd(s, cmul(arow[j], X[j]));
         num = csub(yfin, s);
       }

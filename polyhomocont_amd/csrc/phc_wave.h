@@ -844,29 +844,27 @@ PHC_D cx solve_rows(const cx (&arow)[G], const cx bown, const cdiv_den pdiag,
     if (LP == 64) return pbcast_c<LP>(v, r);  // group 0, lane r
     return gshfl_c<G>(v, r);
   };
+  // Fortran's SUM starts from +0: row i's sum is 0 + p_1 + ... (which is
+  // p_1 unless p_1 has a -0 part); row 0 keeps its right-hand side
   cx sacc = mk(0.0, 0.0);
-  bool sset = false;
   static_for<NC>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    const cx cand = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+    const cx cand = pos == 0 ? bown : csub(bown, sacc);
     const cx yk = rowcast(cand, (int)((plb >> (4 * k)) & 15u));
-    if (row && pos > k) {
-      const cx p = cmul(arow[k], yk);
-      sacc = sset ? cadd(sacc, p) : p;
-      sset = true;
-    }
+    if (row && pos > k) sacc = cadd(sacc, cmul(arow[k], yk));
   });
-  const cx yfin = (pos == 0 || !sset) ? bown : csub(bown, sacc);
+  const cx yfin = pos == 0 ? bown : csub(bown, sacc);
   // back substitution: the row at position i needs
-  // sum_{j > i} a(i, j) x_j in ascending j; each product a(i, j) x_j is
-  // formed once, when x_j is known (px[j]), and step i adds them in order
+  // sum_{j > i} a(i, j) x_j in ascending j, from +0; each product
+  // a(i, j) x_j is formed once, when x_j is known (px[j]), and step i adds
+  // them in order
   cx px[NC];
   cx xmine = mk(0.0, 0.0);
   static_for<NC>([&](auto ic) {
     constexpr int i = NC - 1 - decltype(ic)::value;
     cx num = yfin;
     if constexpr (i + 1 < NC) {
-      cx sm = px[i + 1];
+      cx sm = cadd(mk(0.0, 0.0), px[i + 1]);
       static_for<NC>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         if constexpr (j >= i + 2) sm = cadd(sm, px[j]);
