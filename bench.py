@@ -50,7 +50,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector, dense
-UNITS_PER_LAUNCH = 64
+UNITS_PER_LAUNCH = 256
+LAUNCH_US = 2000
 SUBMITS_PER_CALL = 32
 
 
@@ -97,7 +98,7 @@ class Engine:
     serve() runs a stream and returns per-request results and latencies."""
 
     def __init__(self, ctx, H, workers=4, host_threads=4, resident=16384,
-                 budget=2 << 30, units=UNITS_PER_LAUNCH,
+                 budget=2 << 30, units=UNITS_PER_LAUNCH, launch_us=LAUNCH_US,
                  submits=SUBMITS_PER_CALL, prep_wait_us=500):
         from polyhomocont_amd import device
         self.H = H
@@ -105,7 +106,8 @@ class Engine:
         self.prep_wait_us = prep_wait_us
         self.srv = device.Server(ctx, max_resident_paths=resident,
                                  hbm_budget_bytes=budget,
-                                 units_per_launch=units)
+                                 units_per_launch=units,
+                                 launch_us=launch_us)
         self.pool = ThreadPoolExecutor(
             workers, initializer=lambda: H.set_threads(host_threads))
 
@@ -216,6 +218,7 @@ def serving_section(ctx, H, args, out):
     log(f"serving stream: {len(stream)} requests, {paths} paths")
     eng = Engine(ctx, H, workers=args.host_workers,
                  resident=args.resident_slots, units=args.units_per_launch,
+                 launch_us=args.launch_us,
                  submits=args.submits_per_call,
                  prep_wait_us=args.prep_wait_us)
     for w in range(args.warmup):
@@ -262,6 +265,7 @@ def serving_section(ctx, H, args, out):
                     "classification included",
         "requests": len(stream), "paths": paths,
         "units_per_launch": args.units_per_launch,
+        "launch_us": args.launch_us,
         "resident_path_slots": args.resident_slots,
         "host_pool_workers": args.host_workers}
     out["roofline"] = {
@@ -592,6 +596,9 @@ def main():
     ap.add_argument("--units-per-launch", type=int, default=UNITS_PER_LAUNCH,
                     help="serving engine: linearizations per path slot and "
                          "launch")
+    ap.add_argument("--launch-us", type=int, default=LAUNCH_US,
+                    help="serving engine: time budget of a launch in "
+                         "microseconds (0: the unit budget alone)")
     ap.add_argument("--submits-per-call", type=int, default=SUBMITS_PER_CALL,
                     help="serving engine: requests submitted between two "
                          "progress calls")

@@ -355,7 +355,9 @@ typedef struct phc_server_config {
   int64_t max_resident_paths;  /* path slots kept resident in HBM          */
   int64_t hbm_budget_bytes;    /* admission budget for request state       */
   int32_t units_per_launch;    /* linearizations per path per kernel launch */
-  int32_t reserved;
+  int32_t launch_us;           /* time budget of a launch in microseconds:
+                                  a path slot starts no unit after it (0:
+                                  the unit budget alone bounds launches) */
 } phc_server_config;
 
 int phc_server_create(phc_ctx *ctx, const phc_server_config *cfg,

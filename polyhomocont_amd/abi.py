@@ -103,7 +103,7 @@ class PathResults(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_resident_paths", i64), ("hbm_budget_bytes", i64),
-                ("units_per_launch", i32), ("reserved", i32)]
+                ("units_per_launch", i32), ("launch_us", i32)]
 
 
 def _ptr(a):

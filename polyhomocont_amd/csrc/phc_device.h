@@ -157,7 +157,9 @@ struct Launch {
   int32_t nreq;
   int32_t expmax;           // largest maxexp of the requests
   int32_t vmax;             // grid unit: largest term count of the requests
-  int32_t reserved;
+  int32_t max_ticks;        // resumable launches: a slot starts no unit
+                            // after this many wall-clock ticks of its
+                            // launch (0: no time bound; see max_units)
   unsigned long long* next; // work-queue head
   int64_t total;            // paths in the queue (fixed-size runs)
   // serving: items[i % cap] for head <= i < *tail, appended by the host
@@ -1238,9 +1240,21 @@ __global__ void __launch_bounds__(WG)
   // a unit: units / (passes * paths per wavefront) is the lane occupancy
   // left after path divergence
   int passes = 0;
+  // Resumable launches end on a unit budget and, with max_ticks, on a time
+  // budget: a slot whose units are cheap then keeps working until the
+  // launch's end instead of idling after its unit budget while slots with
+  // expensive units finish theirs. The clock is read every fourth unit;
+  // every slot runs at least one unit per launch. (Results never depend
+  // on where a launch ends: the state is saved and resumed exactly.)
+  const unsigned long long t_launch = L.max_ticks ? wall_clock64() : 0ull;
+  auto budget_spent = [&]() {
+    if (L.max_units && units >= L.max_units) return true;
+    return L.max_ticks && units > 0 && (units & 3) == 0 &&
+           wall_clock64() - t_launch >= (unsigned long long)L.max_ticks;
+  };
   for (;;) {
     if (S.pc == PC_IDLE) {
-      if (L.max_units && units >= L.max_units) break;
+      if (budget_spent()) break;
       // pop the next queue index; -1 when the queue is empty. One
       // fetch-and-add, never retried (a compare-and-swap loop serializes the
       // slots that pop together, at every launch's start, for milliseconds).
@@ -1289,7 +1303,7 @@ __global__ void __launch_bounds__(WG)
       M.start_path((lane < R.N) ? R.start[(size_t)S.path * R.N + lane]
                                 : mk(0.0, 0.0));
     } else {
-      if (L.max_units && units >= L.max_units) break;
+      if (budget_spent()) break;
       const Req& R = STAGED ? *sreq : L.reqs[S.req];
       Machine<G> M{R, S, SV, lane, writer};
       // the unit's inputs, from the path's state in LDS

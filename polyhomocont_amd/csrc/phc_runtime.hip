@@ -41,6 +41,7 @@ struct phc_ctx {
   int mapping = 0;
   int staging = -1;    // program in LDS: -1 when it fits, 0 never
   size_t lds_limit = 64 * 1024;  // largest LDS of one workgroup
+  int wall_khz = 100000;         // rate of the kernels' wall clock (kHz)
   // autotuned launch choices, keyed by the shape of a lanes class (see
   // tune_key): mapping 1-3, workgroup threads, staging (phc_ctx_autotune,
   // phc_ctx_autotune_batch)
@@ -963,6 +964,12 @@ int phc_ctx_create(int device, phc_ctx** out) {
     return PHC_ERR_DEVICE;
   ctx->num_cus = prop.multiProcessorCount;
   ctx->lds_limit = prop.sharedMemPerBlock;
+  {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate,
+                              device) == hipSuccess && khz > 0)
+      ctx->wall_khz = khz;
+  }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
       hipSuccess)
     return PHC_ERR_DEVICE;

@@ -279,10 +279,10 @@ class Server:
     """Continuous-batching serving engine (phc_server_* of include/phc.h)."""
 
     def __init__(self, ctx, max_resident_paths=0, hbm_budget_bytes=0,
-                 units_per_launch=0):
+                 units_per_launch=0, launch_us=0):
         self.ctx = ctx
         cfg = abi.ServerConfig(max_resident_paths, hbm_budget_bytes,
-                               units_per_launch, 0)
+                               units_per_launch, launch_us)
         h = C.c_void_p()
         ctx._check(ctx.L.phc_server_create(ctx.h, C.byref(cfg), C.byref(h)),
                    "phc_server_create")

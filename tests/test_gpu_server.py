@@ -37,13 +37,19 @@ def mixed_requests(host):
     return reqs
 
 
-@pytest.mark.parametrize("budget,units", [(0, 0), (1 << 20, 8)])
-def test_mix_equals_alone_and_plain(ctx, host, budget, units):
+# (budget, units per launch, launch time budget in us): the defaults; a
+# 1 MiB budget with 8-unit launches (admission waits, many resumes); launches
+# ended by a 100 us time budget (a slot stops after a time-dependent number
+# of units, so paths resume at varying places)
+@pytest.mark.parametrize("budget,units,launch_us",
+                         [(0, 0, 0), (1 << 20, 8, 0), (0, 4096, 100)])
+def test_mix_equals_alone_and_plain(ctx, host, budget, units, launch_us):
     reqs = mixed_requests(host)
     plain = [ctx.track_paths(*r) for r in reqs]
     srv = device.Server(ctx, max_resident_paths=2048, hbm_budget_bytes=budget,
-                        units_per_launch=units)
-    print(f"[gpu] server budget={budget} units={units}", flush=True)
+                        units_per_launch=units, launch_us=launch_us)
+    print(f"[gpu] server budget={budget} units={units} launch_us={launch_us}",
+          flush=True)
     # half up front, the rest while the first half is running
     tickets = [srv.submit(*r) for r in reqs[:8]]
     srv.progress()
