@@ -339,6 +339,23 @@ PHC_D cx seg_t(const PathScalars& S, double s) {
   return cscale(S.sb, cs);  // radius in sb
 }
 
+// seg_t with the last angle's cos and sin kept by the caller: RK4's second
+// and third stages share their s, the corrector's iterations share the
+// fourth stage's, and the next step starts where the last one ended, so
+// most units of an arc repeat the previous unit's angle. The pair (kth,
+// kcs) holds theta and sincos(theta) for whichever path last computed
+// them; a NaN kth (the start value) matches no angle.
+template <bool UNIFORM = false>
+PHC_D cx seg_t_cached(const PathScalars& S, double s, double& kth, cx& kcs) {
+  if (S.sk == 0) return seg_t<UNIFORM>(S, s);
+  const double th = S.sa + s * S.sdth;
+  if (!(th == kth)) {
+    kcs = UNIFORM ? libm_sincos_uniform(th) : libm_sincos(th);
+    kth = th;
+  }
+  return cscale(S.sb, kcs);
+}
+
 // -------------------------------------------------------------------- unit
 // LDS of one group: the power table pw(e, j) at T[e * G + j] (e >= 1), the
 // N x N matrix at A[r * G + j], pivot lanes at PL[k] and the solution at
@@ -1252,6 +1269,13 @@ __global__ void __launch_bounds__(WG)
     return L.max_ticks && units > 0 && (units & 3) == 0 &&
            wall_clock64() - t_launch >= (unsigned long long)L.max_ticks;
   };
+  // the last arc angle and its cos and sin (seg_t_cached): 4- and 8-lane
+  // grids with the program in LDS, i.e. single-request solves, whose time
+  // is their longest path's unit latency. The 16-lane kernels have no
+  // registers to spare; in the engine's kernels six more registers cost
+  // about what the saved sincos calls gain.
+  double kth = __builtin_nan("");
+  cx kcs = mk(0.0, 0.0);
   for (;;) {
     if (S.pc == PC_IDLE) {
       if (budget_spent()) break;
@@ -1312,7 +1336,10 @@ __global__ void __launch_bounds__(WG)
         dt = mk(0.0, 0.0);
         if (S.pc != PC_BE) {
           const double sv = (S.pc == PC_TAN) ? S.se : S.sn;
-          t = seg_t<LP == 64>(S, sv);
+          if constexpr (GRID && STAGED && G <= 8)
+            t = seg_t_cached<LP == 64>(S, sv, kth, kcs);
+          else
+            t = seg_t<LP == 64>(S, sv);
           if (S.pc == PC_TAN)
             dt = (S.sk == 0) ? mk(S.sb - S.sa, 0.0)
                              : mk(0.0 * t.re - S.sdth * t.im,
